@@ -1,0 +1,103 @@
+/*
+ * C ABI of the MI355X AIFMM library (librcs_amd.so).
+ *
+ * Plain pointers and sizes only. Complex arrays are interleaved (re, im) doubles and
+ * matrices are column-major. Every call returns 0 on success or a negative status;
+ * rcs_last_error() gives the message of the last failure on the calling thread.
+ * All GPU work runs on one device (the current HIP device at rcs_init) and one stream.
+ *
+ * Each entry point names the reference interface (vaishna77/RCS) it replaces.
+ */
+#ifndef RCS_AMD_H
+#define RCS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rcs_mesh_s* rcs_mesh_t;
+typedef struct rcs_solver_s* rcs_solver_t;
+
+enum { RCS_KERNEL_EFIE = 0, RCS_KERNEL_LAPLACE = 1 };
+enum { RCS_PREC_C128 = 0, RCS_PREC_C64 = 1 };
+
+/* Library set-up: HIP device, stream, HBM block pool. No counterpart in the reference
+ * (it is a host-only program); called implicitly by the other entry points. */
+int rcs_init(void);
+const char* rcs_last_error(void);
+
+/* EFIEMatrix(inputsToKernelClass) (src/EFIEMatrix.cpp:5-20, mesh ingest and RWG
+ * numbering E:31-209) or ElectrostaticsMatrix(inputsToKernelClass)
+ * (src/ElectrostaticsMatrix.cpp:5-9). nodes: n_nodes x 3 (the node file lines);
+ * tris: n_tris x 3, 1-based node indices (the patch file lines); conductor may be NULL.
+ * freq replaces the compile-time constant of include/includes.hpp:58. */
+int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
+                    int kernel_kind, double freq, rcs_mesh_t* out);
+void rcs_mesh_destroy(rcs_mesh_t m);
+/* EFIEMatrix::N (E:97-99) / ElectrostaticsMatrix::N. */
+int rcs_mesh_size(rcs_mesh_t m);
+/* Unknown locations handed to inputsToSolverClass::locations: RWG edge midpoints
+ * (test/sphere.cpp:143-152) or patch centroids. xyz: N x 3. */
+int rcs_mesh_locations(rcs_mesh_t m, double* xyz);
+/* RWG table in basis order: n1 n2 T+ T- free+ free- per basis (edge2PatchMap,
+ * E:129-209 and getIndexOfThirdNode E:353-373), plus the edge lengths. */
+int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length);
+
+/* kernel::getMatrix(rows, cols) (src/kernel.cpp:49-61) over getMatrixEntry
+ * (E:698-783 / src/ElectrostaticsMatrix.cpp:293-340), computed on the GPU in the
+ * requested precision. out: nr x nc complex. */
+int rcs_entries(rcs_mesh_t m, int precision, const int* rows, int nr, const int* cols, int nc, double* out);
+
+/* EFIEMatrix::getRhs (E:785-829): plane wave with field vector E and phase vector
+ * kvec (the reference hard-codes kvec = (0, 0, 1)). out: N complex. */
+int rcs_rhs(rcs_mesh_t m, const double* E, const double* kvec, double* out);
+
+/* getJAtCentroids (E:1100-1130) + getRCS_OnePointQuadrature (E:916-930) at n angles. */
+int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double* phi, int n, double* out);
+
+/* LowRank<K>(K, tol, rows, cols).ACA_only_nodes2 (include/ACA.hpp:16,26; src/ACA.cpp:639-859)
+ * on the GPU. row_bases/col_bases must hold min(nr, nc) entries. */
+int rcs_aca(rcs_mesh_t m, int precision, double tol, const int* rows, int nr, const int* cols, int nc, int* rank,
+            int* row_bases, int* col_bases);
+
+/* RRQR(A, TOL_POW1, TOL_POW2, partition, inputPivot, flag, fillInRank)
+ * .colPivHouseholderQRWithPartition() (include/rrqr.hpp:20-22, src/rrqr.cpp:69-182)
+ * on the GPU. Q must hold m x min(m, n) complex; the first *rank columns are Q_trunc. */
+int rcs_rrqr(int precision, const double* A, int m, int n, int tol_pow1, int tol_pow2, int partition,
+             double input_pivot, int fillin_rank, double* Q, int* rank, double* max_pivot);
+
+/* Eigen::PartialPivLU<Mat>(A).solve(B) as used at E:831-837 and throughout the tree:
+ * blocked GPU GETRF (first-max partial pivoting) + GETRS. */
+int rcs_lu_solve(int precision, const double* A, int n, const double* B, int nrhs, double* X);
+
+/* Dense complex product C = op(A) op(B) (op: 0 none, 1 conjugate transpose) on the GPU,
+ * the operation behind every Mat product of the tree. */
+int rcs_gemm(int precision, int opA, int opB, int m, int n, int k, const double* A, int lda, const double* B,
+             int ldb, double* C);
+
+/* AIFMM<K>(inputsToSolverClass, K*) (src/AIFMM.cpp:6-93): tree set-up, nested ACA
+ * skeletons, L2P/P2M, M2L and leaf P2P assembly, all on the GPU. bbox = xmin xmax ymin
+ * ymax zmin zmax of the locations (test/sphere.cpp:159-174). variant 0 = the compiled
+ * memory-optimised tree, 1 = AIFMMTree2D_3D parameters. */
+int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, int fillin_tol, int fillin_rank,
+                      int precision, int variant, rcs_solver_t* out);
+void rcs_solver_destroy(rcs_solver_t s);
+/* AIFMM<K>::factorize (src/AIFMM.cpp:95-100 -> FMM2DTree::eliminate_phase_efficient T:4599-4747). */
+int rcs_solver_factorize(rcs_solver_t s);
+/* AIFMM<K>::backSubstitute1/2/3 + getPhi (src/AIFMM.cpp:119-147): nrhs right-hand sides
+ * (N x nrhs complex). x gets getPhi's input order; x_box (may be NULL) gets solve()'s
+ * box order (src/AIFMM.cpp:103-110). */
+int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, double* x_box);
+/* Statistics (the driver's prints, test/sphere.cpp:217-262):
+ * info[0..4] = nLevels, startLevel, n_rrqr, n_compress_p2p, n_compress_p2l;
+ * vals[0..10] = t_assemble, t_factor, t_solve, gemm_flops, getrf_flops, getrs_flops, qr_flops,
+ *               t_getrf, t_schur, t_compress, factor_bytes; vals[11] = peak HBM bytes. */
+int rcs_solver_stats(rcs_solver_t s, int* info, double* vals);
+/* Per-box NumMultipoles, NumLocals, particle count at a level (3 ints per box). Returns
+ * the number of boxes written, or a negative status. */
+int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

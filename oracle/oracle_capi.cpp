@@ -1,0 +1,279 @@
+// ORACLE — test infrastructure only (see oracle_core.hpp).
+//
+// Plain C entry points over the FP64 restatement, loaded by the tests (ctypes) as the
+// checker for the HIP library and timed by bench.py as the CPU baseline. Complex
+// arrays are interleaved (re, im) doubles, matrices column-major.
+#include <chrono>
+#include <cstdio>
+#include <memory>
+#include "oracle_tree.hpp"
+
+using namespace oracle;
+
+namespace {
+struct OMesh {
+  Mesh mesh;
+};
+struct OSolver {
+  std::unique_ptr<Kernel> K;
+  std::unique_ptr<AIFMM> S;
+  double t_assemble = 0, t_factor = 0, t_solve = 0;
+};
+std::string g_err;
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+// kind 0: EFIE as the reference codes it; 1: Laplace; 2: EFIE with the singular-term
+// projection on the source plane (test-only, isolates the origin-plane quirk).
+Kernel* makeKernel(const OMesh* m, int kind, double freq) {
+  if (kind == 0 || kind == 2) {
+    EFIE* e = new EFIE(&m->mesh, freq);
+    e->physicalProjection = (kind == 2);
+    return e;
+  }
+  return new Laplace(&m->mesh);
+}
+}  // namespace
+
+#define ORC_TRY try {
+#define ORC_CATCH                       \
+  }                                     \
+  catch (const std::exception& e) {     \
+    g_err = e.what();                   \
+    return -1;                          \
+  }
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_mesh_create(const double* nodes, int nNodes, const int* tris, int nTri, const int* conductor) {
+  OMesh* m = new OMesh;
+  m->mesh.build(nodes, nNodes, tris, nTri, conductor);
+  return m;
+}
+void orc_mesh_destroy(void* h) { delete (OMesh*)h; }
+int orc_mesh_nrwg(void* h) { return (int)((OMesh*)h)->mesh.rwg.size(); }
+int orc_mesh_npatch(void* h) { return (int)((OMesh*)h)->mesh.patches.size(); }
+// per RWG: n1 n2 tp tm fp fm (node indices 1-based as in the files, patches 0-based)
+void orc_mesh_rwg(void* h, int* out6, double* length) {
+  const Mesh& m = ((OMesh*)h)->mesh;
+  for (size_t i = 0; i < m.rwg.size(); ++i) {
+    const RWG& r = m.rwg[i];
+    int* o = out6 + 6 * i;
+    o[0] = r.n1; o[1] = r.n2; o[2] = r.tp; o[3] = r.tm; o[4] = r.fp; o[5] = r.fm;
+    length[i] = r.length;
+  }
+}
+// per patch: centroid xyz, area, largestEdge
+void orc_mesh_patches(void* h, double* out5) {
+  const Mesh& m = ((OMesh*)h)->mesh;
+  for (size_t i = 0; i < m.patches.size(); ++i) {
+    const Patch& p = m.patches[i];
+    double* o = out5 + 5 * i;
+    o[0] = p.centroid.x; o[1] = p.centroid.y; o[2] = p.centroid.z; o[3] = p.area; o[4] = p.largestEdge;
+  }
+}
+
+int orc_entries(void* h, int kind, double freq, const int* rows, int nr, const int* cols, int nc, double* out) {
+  ORC_TRY
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
+  std::vector<int> r(rows, rows + nr), c(cols, cols + nc);
+  Mat M = K->getMatrix(r, c);
+  std::memcpy(out, M.a.data(), sizeof(cd) * M.a.size());
+  return 0;
+  ORC_CATCH
+}
+
+int orc_rhs(void* h, double freq, const double* E, const double* kv, double* out) {
+  ORC_TRY
+  EFIE K(&((OMesh*)h)->mesh, freq);
+  V3 e(E[0], E[1], E[2]), k(kv[0], kv[1], kv[2]);
+  for (int i = 0; i < K.size(); ++i) {
+    cd v = K.rhs(e, k, i);
+    out[2 * i] = v.real();
+    out[2 * i + 1] = v.imag();
+  }
+  return 0;
+  ORC_CATCH
+}
+
+int orc_rcs(void* h, double freq, const double* coef, const double* theta, const double* phi, int n, double* out) {
+  ORC_TRY
+  EFIE K(&((OMesh*)h)->mesh, freq);
+  std::vector<cd> c(K.size());
+  std::memcpy(c.data(), coef, sizeof(cd) * c.size());
+  std::vector<C3> J;
+  K.jAtCentroids(c, J);
+  for (int i = 0; i < n; ++i) out[i] = K.rcs(theta[i], phi[i], J);
+  return 0;
+  ORC_CATCH
+}
+
+// Closed-form panel integrals of 1/R and (r'-p)/R over triangle (a, b, c) seen from tc, p
+// the projection of tc on the triangle's plane
+// (E:478-668), for the quadrature cross-checks.
+int orc_panel(const double* tc, const double* a, const double* b, const double* c, double* lap, double* rho3) {
+  V3 t(tc[0], tc[1], tc[2]), A(a[0], a[1], a[2]), Bv(b[0], b[1], b[2]), Cv(c[0], c[1], c[2]);
+  *lap = analyticLaplaceTri(t, A, Bv, Cv);
+  V3 r = analyticRhoLaplaceTri(t, A, Bv, Cv);
+  rho3[0] = r.x; rho3[1] = r.y; rho3[2] = r.z;
+  return 0;
+}
+
+// bases arrays must hold min(nr, nc) entries
+int orc_aca(void* h, int kind, double freq, const int* rows, int nr, const int* cols, int nc, double tol,
+            int* rank, int* row_bases, int* col_bases) {
+  ORC_TRY
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
+  std::vector<int> r(rows, rows + nr), c(cols, cols + nc);
+  ACAResult res = aca_only_nodes2(*K, tol, r, c);
+  *rank = res.rank;
+  for (int i = 0; i < res.rank; ++i) { row_bases[i] = res.row_bases[i]; col_bases[i] = res.col_bases[i]; }
+  return 0;
+  ORC_CATCH
+}
+
+// Q must hold m*min(m,n) complex values; R must hold min(m,n)*n; perm n ints.
+int orc_rrqr(const double* A, int m, int n, int tolPow1, int tolPow2, int partition, double inputPivot,
+             int fillInRank, double* Q, double* R, int* perm, int* rank, double* maxPivot) {
+  ORC_TRY
+  Mat M(m, n);
+  std::memcpy(M.a.data(), A, sizeof(cd) * M.a.size());
+  RRQRResult r = rrqr(M, tolPow1, tolPow2, partition, inputPivot, fillInRank);
+  *rank = r.rank;
+  *maxPivot = r.maxPivot;
+  std::memcpy(Q, r.Q.a.data(), sizeof(cd) * r.Q.a.size());
+  if (R) std::memcpy(R, r.R.a.data(), sizeof(cd) * r.R.a.size());
+  if (perm)
+    for (int i = 0; i < n; ++i) perm[i] = r.perm[i];
+  return 0;
+  ORC_CATCH
+}
+
+// Dense partial-pivoting LU solve of a caller-supplied matrix (n x n) with nrhs columns.
+int orc_lu_solve(const double* A, int n, const double* B, int nrhs, double* X) {
+  ORC_TRY
+  Mat M(n, n), Bm(n, nrhs);
+  std::memcpy(M.a.data(), A, sizeof(cd) * M.a.size());
+  std::memcpy(Bm.a.data(), B, sizeof(cd) * Bm.a.size());
+  LU lu(M);
+  Mat Xm = lu.solve(Bm);
+  std::memcpy(X, Xm.a.data(), sizeof(cd) * Xm.a.size());
+  return 0;
+  ORC_CATCH
+}
+
+// E:831-837: dense assembly + PartialPivLU solve.
+int orc_dense_solve(void* h, int kind, double freq, const double* rhs, double* x) {
+  ORC_TRY
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
+  int N = K->size();
+  std::vector<int> idx(N);
+  for (int i = 0; i < N; ++i) idx[i] = i;
+  Mat Z = K->getMatrix(idx, idx);
+  Mat b(N, 1);
+  std::memcpy(b.a.data(), rhs, sizeof(cd) * N);
+  LU lu(Z);
+  Mat xx = lu.solve(b);
+  std::memcpy(x, xx.a.data(), sizeof(cd) * N);
+  return 0;
+  ORC_CATCH
+}
+
+// y = Z x with entries generated on the fly (S:331-338 backward-error operator).
+int orc_matvec(void* h, int kind, double freq, const double* x, double* y) {
+  ORC_TRY
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
+  int N = K->size();
+  const cd* xx = (const cd*)x;
+  cd* yy = (cd*)y;
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < N; ++i) {
+    cd s = 0;
+    for (int j = 0; j < N; ++j) s += K->entry(i, j) * xx[j];
+    yy[i] = s;
+  }
+  return 0;
+  ORC_CATCH
+}
+
+void* orc_aifmm_create(void* h, int kind, double freq, const double* loc, int N, const double* bbox, int leafSize,
+                       int tol_pow, int fillin_tol, int fillin_rank, int variant) {
+  try {
+    OSolver* s = new OSolver;
+    s->K.reset(makeKernel((OMesh*)h, kind, freq));
+    SolverInputs in;
+    in.xmin = bbox[0]; in.xmax = bbox[1]; in.ymin = bbox[2]; in.ymax = bbox[3]; in.zmin = bbox[4]; in.zmax = bbox[5];
+    in.leafSize = leafSize;
+    in.tol_pow = tol_pow;
+    in.fillin_tol = fillin_tol;
+    in.fillin_rank = fillin_rank;
+    in.variant = variant;
+    for (int i = 0; i < N; ++i) in.locations.push_back(V3(loc[3 * i], loc[3 * i + 1], loc[3 * i + 2]));
+    double t0 = now();
+    s->S.reset(new AIFMM(in, s->K.get()));
+    s->t_assemble = now() - t0;
+    return s;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_aifmm_destroy(void* h) { delete (OSolver*)h; }
+int orc_aifmm_factorize(void* h) {
+  ORC_TRY
+  OSolver* s = (OSolver*)h;
+  s->S->A->ctr = Counters();
+  double t0 = now();
+  s->S->factorize();
+  s->t_factor = now() - t0;
+  return 0;
+  ORC_CATCH
+}
+// x in the input order (getPhi); x_box (optional) is the solve() result in box order.
+int orc_aifmm_solve(void* h, const double* rhs, double* x, double* x_box) {
+  ORC_TRY
+  OSolver* s = (OSolver*)h;
+  int N = s->S->A->N;
+  std::vector<cd> b(N);
+  std::memcpy(b.data(), rhs, sizeof(cd) * N);
+  s->S->backSubstitute1(b);
+  double t0 = now();
+  s->S->backSubstitute2();
+  s->S->backSubstitute3();
+  s->t_solve = now() - t0;
+  std::vector<cd> xb = s->S->A->getx();
+  if (x_box) std::memcpy(x_box, xb.data(), sizeof(cd) * N);
+  std::vector<cd> p = s->S->getPhi();
+  std::memcpy(x, p.data(), sizeof(cd) * N);
+  return 0;
+  ORC_CATCH
+}
+// info: nLevels, startLevel, n_rrqr, n_compress_p2p, n_compress_p2l
+// times: assemble, factor, solve; flops: gemm, getrf, getrs, qr
+int orc_aifmm_info(void* h, int* info, double* times, double* flops) {
+  OSolver* s = (OSolver*)h;
+  Tree* t = s->S->A;
+  info[0] = t->nLevels;
+  info[1] = t->startLevel;
+  info[2] = (int)t->ctr.n_rrqr;
+  info[3] = (int)t->ctr.n_compress_p2p;
+  info[4] = (int)t->ctr.n_compress_p2l;
+  times[0] = s->t_assemble; times[1] = s->t_factor; times[2] = s->t_solve;
+  flops[0] = t->ctr.gemm_flops; flops[1] = t->ctr.getrf_flops; flops[2] = t->ctr.getrs_flops; flops[3] = t->ctr.qr_flops;
+  return 0;
+}
+// Per-box ranks at level j: NumMultipoles, NumLocals, particle count (3*nBoxes ints).
+int orc_aifmm_ranks(void* h, int j, int* out) {
+  Tree* t = ((OSolver*)h)->S->A;
+  if (j < 0 || j > t->nLevels) return -1;
+  for (int k = 0; k < t->nBoxesPerLevel[j]; ++k) {
+    out[3 * k] = t->B(j, k).NumMultipoles;
+    out[3 * k + 1] = t->B(j, k).NumLocals;
+    out[3 * k + 2] = (int)t->B(j, k).chargeLocations.size();
+  }
+  return t->nBoxesPerLevel[j];
+}
+
+}  // extern "C"

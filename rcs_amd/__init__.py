@@ -1,0 +1,246 @@
+"""MI355X-native AIFMM fast direct solver (EFIE / Laplace single layer).
+
+Python view of the C ABI in include/rcs_amd.h (librcs_amd.so, hand-written HIP for
+gfx950). The classes mirror the reference's C++ interface: ``Mesh`` stands for
+EFIEMatrix / ElectrostaticsMatrix (mesh ingest, getMatrixEntry, getRhs, RCS),
+``AIFMM`` for AIFMM<K> (construction = assembly, factorize, solve), plus the ACA,
+RRQR, LU and GEMM building blocks. Nothing here falls back to a CPU path: a missing
+or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import mesh  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librcs_amd.so")
+_LIB = None
+
+EFIE, LAPLACE = 0, 1
+C128, C64 = 0, 1
+
+
+def lib():
+    """Load librcs_amd.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.rcs_last_error.restype = C.c_char_p
+        L.rcs_mesh_create.argtypes = [dp, C.c_int, ip, C.c_int, ip, C.c_int, C.c_double, C.POINTER(vp)]
+        L.rcs_mesh_destroy.argtypes = [vp]
+        L.rcs_mesh_size.argtypes = [vp]
+        L.rcs_mesh_locations.argtypes = [vp, dp]
+        L.rcs_mesh_rwg.argtypes = [vp, ip, dp]
+        L.rcs_entries.argtypes = [vp, C.c_int, ip, C.c_int, ip, C.c_int, dp]
+        L.rcs_rhs.argtypes = [vp, dp, dp, dp]
+        L.rcs_rcs.argtypes = [vp, dp, dp, dp, C.c_int, dp]
+        L.rcs_aca.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, ip, ip, ip]
+        L.rcs_rrqr.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                               dp, ip, dp]
+        L.rcs_lu_solve.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp]
+        L.rcs_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp]
+        L.rcs_solver_create.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(vp)]
+        L.rcs_solver_destroy.argtypes = [vp]
+        L.rcs_solver_factorize.argtypes = [vp]
+        L.rcs_solver_solve.argtypes = [vp, dp, C.c_int, dp, dp]
+        L.rcs_solver_stats.argtypes = [vp, ip, dp]
+        L.rcs_solver_ranks.argtypes = [vp, C.c_int, ip, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+EXPORTED = [
+    "rcs_init", "rcs_last_error", "rcs_mesh_create", "rcs_mesh_destroy", "rcs_mesh_size", "rcs_mesh_locations",
+    "rcs_mesh_rwg", "rcs_entries", "rcs_rhs", "rcs_rcs", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
+    "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
+    "rcs_solver_ranks",
+]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("rcs_amd: " + lib().rcs_last_error().decode())
+
+
+class Mesh:
+    """EFIEMatrix / ElectrostaticsMatrix over the GPU geometry tables."""
+
+    def __init__(self, nodes, tris, kind=EFIE, freq=1.0e9, conductor=None):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        tris = np.ascontiguousarray(tris, dtype=np.int32)
+        cond = None if conductor is None else np.ascontiguousarray(conductor, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().rcs_mesh_create(_dp(nodes), nodes.shape[0], _ip(tris), tris.shape[0],
+                                     None if cond is None else _ip(cond), kind, freq, C.byref(h)))
+        self.h = h
+        self.kind = kind
+        self.freq = freq
+        self.N = lib().rcs_mesh_size(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rcs_mesh_destroy(self.h)
+            self.h = None
+
+    def locations(self):
+        out = np.zeros((self.N, 3))
+        _check(lib().rcs_mesh_locations(self.h, _dp(out)))
+        return out
+
+    def rwg(self):
+        t = np.zeros((self.N, 6), dtype=np.int32)
+        ln = np.zeros(self.N)
+        _check(lib().rcs_mesh_rwg(self.h, _ip(t), _dp(ln)))
+        return t, ln
+
+    def entries(self, rows, cols, precision=C128):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.zeros(rows.size * cols.size, dtype=np.complex128)
+        _check(lib().rcs_entries(self.h, precision, _ip(rows), rows.size, _ip(cols), cols.size,
+                                 out.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(cols.size, rows.size).T
+
+    def rhs(self, E=(1.0, 0.0, 0.0), kvec=(0.0, 0.0, 1.0)):
+        E = np.asarray(E, dtype=np.float64)
+        k = np.asarray(kvec, dtype=np.float64)
+        out = np.zeros(self.N, dtype=np.complex128)
+        _check(lib().rcs_rhs(self.h, _dp(E), _dp(k), out.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def rcs(self, coef, theta, phi):
+        coef = np.ascontiguousarray(coef, dtype=np.complex128)
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.zeros(theta.size)
+        _check(lib().rcs_rcs(self.h, coef.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), _dp(theta),
+                             _dp(phi), theta.size, _dp(out)))
+        return out
+
+    def aca(self, rows, cols, tol, precision=C128):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        m = max(1, min(rows.size, cols.size))
+        rk = np.zeros(1, dtype=np.int32)
+        rb = np.zeros(m, dtype=np.int32)
+        cb = np.zeros(m, dtype=np.int32)
+        _check(lib().rcs_aca(self.h, precision, tol, _ip(rows), rows.size, _ip(cols), cols.size, _ip(rk), _ip(rb),
+                             _ip(cb)))
+        r = int(rk[0])
+        return r, rb[:r].copy(), cb[:r].copy()
+
+
+def _cplx_ptr(a):
+    return a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def rrqr(A, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, precision=C128):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m, n = A.shape
+    mn = max(1, min(m, n))
+    Q = np.zeros(m * mn, dtype=np.complex128)
+    rk = np.zeros(1, dtype=np.int32)
+    mp = np.zeros(1)
+    Af = A.reshape(-1, order="F").copy()
+    _check(lib().rcs_rrqr(precision, _cplx_ptr(Af), m, n, tol_pow1, tol_pow2, partition,
+                          input_pivot, fillin_rank, _cplx_ptr(Q), _ip(rk), _dp(mp)))
+    r = int(rk[0])
+    return Q[: m * r].reshape((m, r), order="F"), r, float(mp[0])
+
+
+def lu_solve(A, B, precision=C128):
+    A = np.asarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    B2 = np.asarray(B, dtype=np.complex128).reshape(n, -1)
+    Af = A.reshape(-1, order="F").copy()
+    Bf = B2.reshape(-1, order="F").copy()
+    X = np.zeros_like(Bf)
+    _check(lib().rcs_lu_solve(precision, _cplx_ptr(Af), n, _cplx_ptr(Bf), B2.shape[1], _cplx_ptr(X)))
+    return X.reshape(B2.shape, order="F")
+
+
+def gemm(A, B, opA=0, opB=0, precision=C128):
+    A = np.asarray(A, dtype=np.complex128)
+    B = np.asarray(B, dtype=np.complex128)
+    m = A.shape[0] if opA == 0 else A.shape[1]
+    k = A.shape[1] if opA == 0 else A.shape[0]
+    n = B.shape[1] if opB == 0 else B.shape[0]
+    Af = A.reshape(-1, order="F").copy()
+    Bf = B.reshape(-1, order="F").copy()
+    Cf = np.zeros(m * n, dtype=np.complex128)
+    _check(lib().rcs_gemm(precision, opA, opB, m, n, k, _cplx_ptr(Af), A.shape[0], _cplx_ptr(Bf), B.shape[0],
+                          _cplx_ptr(Cf)))
+    return Cf.reshape((m, n), order="F")
+
+
+class AIFMM:
+    """AIFMM<K>: construction assembles the compressed operator on the GPU."""
+
+    def __init__(self, mesh_: Mesh, bbox, leaf_size=64, tol_pow=6, fillin_tol=6, fillin_rank=0, precision=C128,
+                 variant=0):
+        bbox = np.ascontiguousarray(bbox, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().rcs_solver_create(mesh_.h, _dp(bbox), leaf_size, tol_pow, fillin_tol, fillin_rank, precision,
+                                       variant, C.byref(h)))
+        self.h = h
+        self.mesh = mesh_
+        self.N = mesh_.N
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rcs_solver_destroy(self.h)
+            self.h = None
+
+    def factorize(self):
+        _check(lib().rcs_solver_factorize(self.h))
+
+    def solve(self, rhs, box_order=False):
+        rhs = np.asarray(rhs, dtype=np.complex128)
+        single = rhs.ndim == 1
+        R = rhs.reshape(self.N, -1)
+        nrhs = R.shape[1]
+        Rf = R.reshape(-1, order="F").copy()
+        X = np.zeros_like(Rf)
+        XB = np.zeros_like(Rf)
+        _check(lib().rcs_solver_solve(self.h, _cplx_ptr(Rf), nrhs, _cplx_ptr(X), _cplx_ptr(XB)))
+        X = X.reshape((self.N, nrhs), order="F")
+        XB = XB.reshape((self.N, nrhs), order="F")
+        if single:
+            X, XB = X[:, 0], XB[:, 0]
+        return (X, XB) if box_order else X
+
+    def stats(self):
+        i = np.zeros(5, dtype=np.int32)
+        v = np.zeros(12)
+        _check(lib().rcs_solver_stats(self.h, _ip(i), _dp(v)))
+        keys = ["t_assemble", "t_factor", "t_solve", "gemm_flops", "getrf_flops", "getrs_flops", "qr_flops",
+                "t_getrf", "t_schur", "t_compress", "factor_bytes", "peak_bytes"]
+        d = {"nLevels": int(i[0]), "startLevel": int(i[1]), "n_rrqr": int(i[2]), "n_compress_p2p": int(i[3]),
+             "n_compress_p2l": int(i[4])}
+        d.update({k: float(x) for k, x in zip(keys, v)})
+        return d
+
+    def ranks(self, level):
+        cap = 3 * 8 ** level
+        out = np.zeros(cap, dtype=np.int32)
+        n = lib().rcs_solver_ranks(self.h, level, _ip(out), cap)
+        if n < 0:
+            _check(n)
+        return out[: 3 * n].reshape(n, 3)

@@ -1,0 +1,390 @@
+// extern "C" boundary of librcs_amd.so (declarations and reference citations in
+// include/rcs_amd.h).
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "../../include/rcs_amd.h"
+#include "aca.hpp"
+#include "ctx.hpp"
+#include "dense.hpp"
+#include "entries.hpp"
+#include "geometry.hpp"
+#include "tree.hpp"
+
+using namespace rcs;
+
+struct rcs_mesh_s {
+  std::shared_ptr<DeviceGeometry> g;
+};
+struct rcs_solver_s {
+  std::unique_ptr<SolverBase> s;
+  std::shared_ptr<DeviceGeometry> g;
+};
+
+namespace {
+thread_local std::string g_err;
+std::unique_ptr<Ctx> g_ctx;
+std::mutex g_mu;
+
+Ctx& ctx() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx) {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (n <= 0) throw std::runtime_error("no HIP device");
+    g_ctx.reset(new Ctx());
+  }
+  return *g_ctx;
+}
+int fail(const std::exception& e) {
+  g_err = e.what();
+  return -1;
+}
+template <typename T>
+DMat<T> dalloc(Ctx& cx, int r, int c) {
+  DMat<T> m;
+  m.r = r;
+  m.c = c;
+  m.ld = std::max(r, 1);
+  m.p = (T*)cx.pool.alloc(sizeof(T) * std::max<size_t>((size_t)r * c, 1));
+  return m;
+}
+// host complex128 -> device T
+template <typename T>
+DMat<T> upload(Ctx& cx, const double* h, int r, int c, int ldh);
+template <>
+DMat<cpx<double>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
+  DMat<cpx<double>> m = dalloc<cpx<double>>(cx, r, c);
+  HIPCHK(hipMemcpy2DAsync(m.p, sizeof(cpx<double>) * m.ld, h, sizeof(cpx<double>) * ldh, sizeof(cpx<double>) * r, c,
+                          hipMemcpyHostToDevice, cx.st));
+  return m;
+}
+template <>
+DMat<cpx<float>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
+  DMat<cpx<double>> t = upload<cpx<double>>(cx, h, r, c, ldh);
+  DMat<cpx<float>> m = dalloc<cpx<float>>(cx, r, c);
+  cast_c128_to_c64(m, t, cx.st);
+  cx.sync();
+  cx.pool.free(t.p);
+  return m;
+}
+template <typename T>
+void download(Ctx& cx, DMat<T> m, double* h);
+template <>
+void download(Ctx& cx, DMat<cpx<double>> m, double* h) {
+  HIPCHK(hipMemcpy2DAsync(h, sizeof(cpx<double>) * m.r, m.p, sizeof(cpx<double>) * m.ld, sizeof(cpx<double>) * m.r,
+                          m.c, hipMemcpyDeviceToHost, cx.st));
+  cx.sync();
+}
+template <>
+void download(Ctx& cx, DMat<cpx<float>> m, double* h) {
+  DMat<cpx<double>> t = dalloc<cpx<double>>(cx, m.r, m.c);
+  cast_c64_to_c128(t, m, cx.st);
+  download<cpx<double>>(cx, t, h);
+  cx.pool.free(t.p);
+}
+
+template <typename T>
+void entries_t(rcs_mesh_t m, const int* rows, int nr, const int* cols, int nc, double* out) {
+  Ctx& cx = ctx();
+  DMat<T> Z = dalloc<T>(cx, nr, nc);
+  if (nr > 0 && nc > 0) {
+    EntryTask<T> t{};
+    t.rows = (const int*)cx.push(rows, sizeof(int) * nr);
+    t.cols = (const int*)cx.push(cols, sizeof(int) * nc);
+    t.nr = nr;
+    t.nc = nc;
+    t.out = Z.p;
+    t.ld = Z.ld;
+    t.tile0 = 0;
+    const EntryTask<T>* d = (const EntryTask<T>*)cx.push(&t, sizeof(t));
+    launch_entries<T>(m->g->d, m->g->h.kind, d, 1, entry_tiles(nr, nc), cx.st);
+    download<T>(cx, Z, out);
+  }
+  cx.pool.free(Z.p);
+}
+
+template <typename T>
+void aca_t(rcs_mesh_t m, double tol, const int* rows, int nr, const int* cols, int nc, int* rank, int* rb, int* cb) {
+  Ctx& cx = ctx();
+  std::vector<AcaProblem> probs(1);
+  probs[0].rows.assign(rows, rows + nr);
+  probs[0].cols.assign(cols, cols + nc);
+  std::vector<AcaResult> res;
+  aca_batched<T>(m->g->d, m->g->h.kind, tol, probs, res, cx);
+  *rank = res[0].rank;
+  for (int i = 0; i < res[0].rank; ++i) {
+    rb[i] = probs[0].rows.empty() ? 0 : res[0].row_bases[i];
+    cb[i] = res[0].col_bases[i];
+  }
+}
+
+template <typename T>
+void rrqr_t(const double* A, int m, int n, int tp1, int tp2, int partition, double ip, int fr, double* Q, int* rank,
+            double* mp) {
+  Ctx& cx = ctx();
+  DMat<T> dA = upload<T>(cx, A, m, n, m);
+  int mn = std::max(1, std::min(m, n));
+  DMat<T> dQ = dalloc<T>(cx, m, mn);
+  RRQRInfo inf = rrqr(dA, partition, std::pow(10.0, -tp1), std::pow(10.0, -tp2), ip, fr, dQ, cx);
+  *rank = inf.rank;
+  *mp = inf.maxPivot;
+  if (inf.rank > 0) download<T>(cx, dQ.cols(0, inf.rank), Q);
+  cx.pool.free(dA.p);
+  cx.pool.free(dQ.p);
+}
+
+template <typename T>
+void lu_t(const double* A, int n, const double* B, int nrhs, double* X) {
+  Ctx& cx = ctx();
+  DMat<T> dA = upload<T>(cx, A, n, n, n);
+  DMat<T> dB = upload<T>(cx, B, n, nrhs, n);
+  int* ip = (int*)cx.pool.alloc(sizeof(int) * std::max(n, 1));
+  getrf(dA, ip, cx);
+  getrs(dA, ip, dB, cx);
+  download<T>(cx, dB, X);
+  cx.pool.free(dA.p);
+  cx.pool.free(dB.p);
+  cx.pool.free(ip);
+}
+
+template <typename T>
+void gemm_t(int opA, int opB, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C) {
+  Ctx& cx = ctx();
+  int ar = opA == OP_N ? m : k, ac = opA == OP_N ? k : m;
+  int br = opB == OP_N ? k : n, bc = opB == OP_N ? n : k;
+  DMat<T> dA = upload<T>(cx, A, ar, ac, lda);
+  DMat<T> dB = upload<T>(cx, B, br, bc, ldb);
+  DMat<T> dC = dalloc<T>(cx, m, n);
+  std::vector<GemmOp<T>> g(1);
+  g[0] = GemmOp<T>{};
+  g[0].A = dA.p; g[0].lda = dA.ld; g[0].opA = opA;
+  g[0].B = dB.p; g[0].ldb = dB.ld; g[0].opB = opB;
+  g[0].C = dC.p; g[0].ldc = dC.ld;
+  g[0].m = m; g[0].n = n; g[0].k = k;
+  g[0].alpha = T(1, 0); g[0].beta = T(0, 0);
+  if (k == 0) zero2d(dC, cx.st);
+  else gemm_grouped(g, cx);
+  download<T>(cx, dC, C);
+  cx.pool.free(dA.p);
+  cx.pool.free(dB.p);
+  cx.pool.free(dC.p);
+}
+}  // namespace
+
+#define TRY try {
+#define CATCH                              \
+  }                                        \
+  catch (const std::exception& e) {        \
+    return fail(e);                        \
+  }
+
+extern "C" {
+
+int rcs_init(void) {
+  TRY ctx();
+  return 0;
+  CATCH
+}
+const char* rcs_last_error(void) { return g_err.c_str(); }
+
+int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
+                    int kernel_kind, double freq, rcs_mesh_t* out) {
+  TRY if (kernel_kind != KIND_EFIE && kernel_kind != KIND_LAPLACE) throw std::runtime_error("unknown kernel kind");
+  if (!(freq > 0)) throw std::runtime_error("frequency must be positive");
+  std::unique_ptr<rcs_mesh_s> m(new rcs_mesh_s);
+  m->g = std::make_shared<DeviceGeometry>();
+  m->g->h.kind = kernel_kind;
+  m->g->h.freq = freq;
+  m->g->h.build(nodes, n_nodes, tris, n_tris, conductor);
+  m->g->upload(ctx());
+  *out = m.release();
+  return 0;
+  CATCH
+}
+void rcs_mesh_destroy(rcs_mesh_t m) { delete m; }
+int rcs_mesh_size(rcs_mesh_t m) { return m ? m->g->h.n_unknowns() : -1; }
+int rcs_mesh_locations(rcs_mesh_t m, double* xyz) {
+  TRY std::vector<double> v;
+  m->g->h.locations(v);
+  std::memcpy(xyz, v.data(), sizeof(double) * v.size());
+  return 0;
+  CATCH
+}
+int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length) {
+  TRY const HostGeometry& h = m->g->h;
+  for (size_t i = 0; i < h.n1.size(); ++i) {
+    int* o = out6 + 6 * i;
+    o[0] = h.n1[i]; o[1] = h.n2[i]; o[2] = h.tp[i]; o[3] = h.tm[i]; o[4] = h.fp[i]; o[5] = h.fm[i];
+    length[i] = h.len[i];
+  }
+  return 0;
+  CATCH
+}
+
+int rcs_entries(rcs_mesh_t m, int precision, const int* rows, int nr, const int* cols, int nc, double* out) {
+  TRY const int N = m->g->h.n_unknowns();
+  for (int i = 0; i < nr; ++i) if (rows[i] < 0 || rows[i] >= N) throw std::runtime_error("row index out of range");
+  for (int i = 0; i < nc; ++i) if (cols[i] < 0 || cols[i] >= N) throw std::runtime_error("column index out of range");
+  if (precision == PREC_C128) entries_t<cpx<double>>(m, rows, nr, cols, nc, out);
+  else if (precision == PREC_C64) entries_t<cpx<float>>(m, rows, nr, cols, nc, out);
+  else throw std::runtime_error("unknown precision");
+  return 0;
+  CATCH
+}
+
+int rcs_rhs(rcs_mesh_t m, const double* E, const double* kvec, double* out) {
+  TRY const HostGeometry& h = m->g->h;
+  if (h.kind != KIND_EFIE) throw std::runtime_error("rcs_rhs needs an EFIE mesh");
+  for (size_t i = 0; i < h.n1.size(); ++i) {
+    int P = h.tp[i], M = h.tm[i];
+    double rp[3] = {h.cx[P] - h.nx[h.fp[i]], h.cy[P] - h.ny[h.fp[i]], h.cz[P] - h.nz[h.fp[i]]};
+    double rm[3] = {h.cx[M] - h.nx[h.fm[i]], h.cy[M] - h.ny[h.fm[i]], h.cz[M] - h.nz[h.fm[i]]};
+    double kp = kvec[0] * h.cx[P] + kvec[1] * h.cy[P] + kvec[2] * h.cz[P];
+    double km = kvec[0] * h.cx[M] + kvec[1] * h.cy[M] + kvec[2] * h.cz[M];
+    double ep = E[0] * rp[0] + E[1] * rp[1] + E[2] * rp[2];
+    double em = E[0] * rm[0] + E[1] * rm[1] + E[2] * rm[2];
+    double re = ep * std::cos(kp) - em * std::cos(km), im = ep * std::sin(kp) - em * std::sin(km);
+    out[2 * i] = h.len[i] / 2.0 * re;
+    out[2 * i + 1] = h.len[i] / 2.0 * im;
+  }
+  return 0;
+  CATCH
+}
+
+int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double* phi, int n, double* out) {
+  TRY const HostGeometry& h = m->g->h;
+  if (h.kind != KIND_EFIE) throw std::runtime_error("rcs_rcs needs an EFIE mesh");
+  const size_t F = h.v1.size();
+  std::vector<double> J(6 * F, 0.0);  // (Jx, Jy, Jz) complex per patch
+  for (size_t i = 0; i < h.n1.size(); ++i) {
+    double cr = coef[2 * i], ci = coef[2 * i + 1];
+    int P = h.tp[i], M = h.tm[i];
+    double sp = h.len[i] / 2 / h.area[P], sm = -h.len[i] / 2 / h.area[M];
+    double rp[3] = {h.cx[P] - h.nx[h.fp[i]], h.cy[P] - h.ny[h.fp[i]], h.cz[P] - h.nz[h.fp[i]]};
+    double rm[3] = {h.cx[M] - h.nx[h.fm[i]], h.cy[M] - h.ny[h.fm[i]], h.cz[M] - h.nz[h.fm[i]]};
+    for (int d = 0; d < 3; ++d) {
+      J[6 * P + 2 * d] += cr * rp[d] * sp;
+      J[6 * P + 2 * d + 1] += ci * rp[d] * sp;
+      J[6 * M + 2 * d] += cr * rm[d] * sm;
+      J[6 * M + 2 * d + 1] += ci * rm[d] * sm;
+    }
+  }
+  const double PI = 4.0 * std::atan(1.0);
+  const double kap = 2 * PI / (3.0e8 / h.freq);
+  const int neta = 377;  // E:917
+  for (int a = 0; a < n; ++a) {
+    double th = theta[a], ph = phi[a];
+    double kv[3] = {kap * std::sin(th) * std::cos(ph), kap * std::sin(th) * std::sin(ph), kap * std::cos(th)};
+    double str = 0, sti = 0, spr = 0, spi = 0;
+    for (size_t t = 0; t < F; ++t) {
+      double ph_ = kv[0] * h.cx[t] + kv[1] * h.cy[t] + kv[2] * h.cz[t];
+      double c = std::cos(ph_), s = std::sin(ph_), A = h.area[t];
+      const double* j = &J[6 * t];
+      double ar = j[0] * std::cos(th) * std::cos(ph) + j[2] * std::cos(th) * std::sin(ph) - j[4] * std::sin(th);
+      double ai = j[1] * std::cos(th) * std::cos(ph) + j[3] * std::cos(th) * std::sin(ph) - j[5] * std::sin(th);
+      str += (ar * c - ai * s) * A;
+      sti += (ar * s + ai * c) * A;
+      double br = -j[0] * std::sin(ph) + j[2] * std::cos(ph), bi = -j[1] * std::sin(ph) + j[3] * std::cos(ph);
+      spr += (br * c - bi * s) * A;
+      spi += (br * s + bi * c) * A;
+    }
+    double at = std::sqrt(str * str + sti * sti), ap = std::sqrt(spr * spr + spi * spi);
+    out[a] = kap * kap / (4.0 * PI) * neta * at * neta * at + kap * kap / (4.0 * PI) * neta * ap * neta * ap;
+  }
+  return 0;
+  CATCH
+}
+
+int rcs_aca(rcs_mesh_t m, int precision, double tol, const int* rows, int nr, const int* cols, int nc, int* rank,
+            int* row_bases, int* col_bases) {
+  TRY if (nr <= 0 || nc <= 0) { *rank = 0; return 0; }
+  if (precision == PREC_C128) aca_t<cpx<double>>(m, tol, rows, nr, cols, nc, rank, row_bases, col_bases);
+  else if (precision == PREC_C64) aca_t<cpx<float>>(m, tol, rows, nr, cols, nc, rank, row_bases, col_bases);
+  else throw std::runtime_error("unknown precision");
+  return 0;
+  CATCH
+}
+
+int rcs_rrqr(int precision, const double* A, int m, int n, int tol_pow1, int tol_pow2, int partition,
+             double input_pivot, int fillin_rank, double* Q, int* rank, double* max_pivot) {
+  TRY if (partition < 0 || partition > n) throw std::runtime_error("partition out of range");
+  if (m <= 0 || n <= 0) { *rank = 0; *max_pivot = 0; return 0; }
+  if (precision == PREC_C128) rrqr_t<cpx<double>>(A, m, n, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, Q, rank, max_pivot);
+  else if (precision == PREC_C64) rrqr_t<cpx<float>>(A, m, n, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, Q, rank, max_pivot);
+  else throw std::runtime_error("unknown precision");
+  return 0;
+  CATCH
+}
+
+int rcs_lu_solve(int precision, const double* A, int n, const double* B, int nrhs, double* X) {
+  TRY if (n <= 0 || nrhs <= 0) return 0;
+  if (precision == PREC_C128) lu_t<cpx<double>>(A, n, B, nrhs, X);
+  else if (precision == PREC_C64) lu_t<cpx<float>>(A, n, B, nrhs, X);
+  else throw std::runtime_error("unknown precision");
+  return 0;
+  CATCH
+}
+
+int rcs_gemm(int precision, int opA, int opB, int m, int n, int k, const double* A, int lda, const double* B,
+             int ldb, double* C) {
+  TRY if (m <= 0 || n <= 0) return 0;
+  if ((opA != 0 && opA != 1) || (opB != 0 && opB != 1)) throw std::runtime_error("op must be 0 or 1");
+  if (precision == PREC_C128) gemm_t<cpx<double>>(opA, opB, m, n, k, A, lda, B, ldb, C);
+  else if (precision == PREC_C64) gemm_t<cpx<float>>(opA, opB, m, n, k, A, lda, B, ldb, C);
+  else throw std::runtime_error("unknown precision");
+  return 0;
+  CATCH
+}
+
+int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, int fillin_tol, int fillin_rank,
+                      int precision, int variant, rcs_solver_t* out) {
+  TRY if (leaf_size <= 0) throw std::runtime_error("leaf size must be positive");
+  SolverParams p;
+  for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];
+  p.leafSize = leaf_size;
+  p.tol_pow = tol_pow;
+  p.fillin_tol = fillin_tol;
+  p.fillin_rank = fillin_rank;
+  p.variant = variant;
+  std::unique_ptr<rcs_solver_s> s(new rcs_solver_s);
+  s->g = m->g;
+  s->s = make_solver(m->g, p, precision, ctx());
+  s->s->assemble();
+  *out = s.release();
+  return 0;
+  CATCH
+}
+void rcs_solver_destroy(rcs_solver_t s) { delete s; }
+int rcs_solver_factorize(rcs_solver_t s) {
+  TRY s->s->factorize();
+  return 0;
+  CATCH
+}
+int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, double* x_box) {
+  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  s->s->solve(rhs, nrhs, x, x_box);
+  return 0;
+  CATCH
+}
+int rcs_solver_stats(rcs_solver_t s, int* info, double* vals) {
+  TRY SolverStats st = s->s->stats();
+  info[0] = st.nLevels; info[1] = st.startLevel; info[2] = (int)st.n_rrqr; info[3] = (int)st.n_compress_p2p;
+  info[4] = (int)st.n_compress_p2l;
+  vals[0] = st.t_assemble; vals[1] = st.t_factor; vals[2] = st.t_solve; vals[3] = st.gemm_flops;
+  vals[4] = st.getrf_flops; vals[5] = st.getrs_flops; vals[6] = st.qr_flops; vals[7] = st.t_getrf;
+  vals[8] = st.t_schur; vals[9] = st.t_compress; vals[10] = st.factor_bytes; vals[11] = (double)st.peak_bytes;
+  return 0;
+  CATCH
+}
+int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap) {
+  TRY std::vector<int> v;
+  s->s->ranks(level, v);
+  if ((int)v.size() > cap) throw std::runtime_error("rank buffer too small");
+  std::memcpy(out, v.data(), sizeof(int) * v.size());
+  return (int)v.size() / 3;
+  CATCH
+}
+
+}  // extern "C"

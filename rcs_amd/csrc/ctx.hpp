@@ -1,0 +1,124 @@
+// Execution context: one HIP stream, a caching HBM block pool and a staging ring for
+// launch parameter arrays (pinned host memory mirrored into HBM in stream order).
+#pragma once
+#include <cstring>
+#include <map>
+
+#include "rcs_common.hpp"
+
+namespace rcs {
+
+// Best-fit caching allocator over hipMalloc'd blocks. The factorization creates and
+// drops thousands of blocks whose sizes repeat (box x box, rank x box, rank x rank);
+// cached blocks are reused instead of going back to the driver.
+class DevPool {
+ public:
+  ~DevPool() { release_all(); }
+  void* alloc(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= bytes + bytes / 4) {
+      void* p = it->second;
+      size_t sz = it->first;
+      free_.erase(it);
+      used_[p] = sz;
+      in_use_ += sz;
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      trim();
+      HIPCHK(hipMalloc(&p, bytes));
+    }
+    used_[p] = bytes;
+    total_ += bytes;
+    in_use_ += bytes;
+    peak_ = std::max(peak_, in_use_);
+    return p;
+  }
+  void free(void* p) {
+    if (!p) return;
+    auto it = used_.find(p);
+    if (it == used_.end()) throw std::runtime_error("DevPool: free of unknown pointer");
+    in_use_ -= it->second;
+    free_.insert({it->second, p});
+    used_.erase(it);
+  }
+  void trim() {
+    for (auto& kv : free_) { (void)hipFree(kv.second); total_ -= kv.first; }
+    free_.clear();
+  }
+  void release_all() {
+    trim();
+    for (auto& kv : used_) (void)hipFree(kv.first);
+    used_.clear();
+    total_ = in_use_ = 0;
+  }
+  size_t in_use() const { return in_use_; }
+  size_t peak() const { return peak_; }
+  size_t total() const { return total_; }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> used_;
+  size_t total_ = 0, in_use_ = 0, peak_ = 0;
+};
+
+struct Ctx {
+  hipStream_t st = nullptr;
+  DevPool pool;
+  char* h_stage = nullptr;
+  char* d_stage = nullptr;
+  size_t cap = 0, off = 0;
+  double* d_scalars = nullptr;  // scratch for reductions read back by the host
+  double* h_scalars = nullptr;
+  size_t n_scalars = 0;
+
+  explicit Ctx(size_t stage_bytes = size_t(64) << 20, size_t nscal = 1 << 16) {
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    cap = stage_bytes;
+    HIPCHK(hipHostMalloc((void**)&h_stage, cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&d_stage, cap));
+    n_scalars = nscal;
+    HIPCHK(hipMalloc((void**)&d_scalars, sizeof(double) * nscal));
+    HIPCHK(hipHostMalloc((void**)&h_scalars, sizeof(double) * nscal, hipHostMallocDefault));
+  }
+  ~Ctx() {
+    if (st) (void)hipStreamSynchronize(st);
+    pool.release_all();
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (d_stage) (void)hipFree(d_stage);
+    if (d_scalars) (void)hipFree(d_scalars);
+    if (h_scalars) (void)hipHostFree(h_scalars);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  Ctx(const Ctx&) = delete;
+  Ctx& operator=(const Ctx&) = delete;
+
+  // Copy a small parameter array to HBM in stream order; returns the device address.
+  void* push(const void* data, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes > cap) throw std::runtime_error("Ctx::push: parameter block larger than the staging ring");
+    if (off + bytes > cap) {
+      HIPCHK(hipStreamSynchronize(st));  // every earlier copy out of the ring has completed
+      off = 0;
+    }
+    std::memcpy(h_stage + off, data, bytes);
+    void* d = d_stage + off;
+    HIPCHK(hipMemcpyAsync(d, h_stage + off, bytes, hipMemcpyHostToDevice, st));
+    off += bytes;
+    return d;
+  }
+  void sync() { HIPCHK(hipStreamSynchronize(st)); }
+  // Fetch n doubles of d_scalars (synchronizes).
+  const double* fetch_scalars(size_t n) {
+    if (n > n_scalars) throw std::runtime_error("fetch_scalars: too many");
+    HIPCHK(hipMemcpyAsync(h_scalars, d_scalars, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    sync();
+    return h_scalars;
+  }
+};
+
+}  // namespace rcs

@@ -1,0 +1,1353 @@
+// GPU AIFMM tree: see tree.hpp. Function names and the order of operations follow
+// include/AIFMMTree2D_3D_memory_optimized.hpp (cited as T:line) so that the elimination
+// takes the same decisions (ranks, fill-in pairs, compression order) as the reference.
+#include "tree.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace rcs {
+
+static double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void DeviceGeometry::upload(Ctx& c) {
+  cx = &c;
+  auto up = [&](const void* src, size_t bytes) -> void* {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes ? bytes : 16));
+    if (bytes) HIPCHK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    bufs.push_back(p);
+    return p;
+  };
+  d.nx = (const double*)up(h.nx.data(), sizeof(double) * h.nx.size());
+  d.ny = (const double*)up(h.ny.data(), sizeof(double) * h.ny.size());
+  d.nz = (const double*)up(h.nz.data(), sizeof(double) * h.nz.size());
+  d.v1 = (const int*)up(h.v1.data(), sizeof(int) * h.v1.size());
+  d.v2 = (const int*)up(h.v2.data(), sizeof(int) * h.v2.size());
+  d.v3 = (const int*)up(h.v3.data(), sizeof(int) * h.v3.size());
+  d.cx = (const double*)up(h.cx.data(), sizeof(double) * h.cx.size());
+  d.cy = (const double*)up(h.cy.data(), sizeof(double) * h.cy.size());
+  d.cz = (const double*)up(h.cz.data(), sizeof(double) * h.cz.size());
+  d.area = (const double*)up(h.area.data(), sizeof(double) * h.area.size());
+  d.le = (const double*)up(h.le.data(), sizeof(double) * h.le.size());
+  d.tp = (const int*)up(h.tp.data(), sizeof(int) * h.tp.size());
+  d.tm = (const int*)up(h.tm.data(), sizeof(int) * h.tm.size());
+  d.fp = (const int*)up(h.fp.data(), sizeof(int) * h.fp.size());
+  d.fm = (const int*)up(h.fm.data(), sizeof(int) * h.fm.size());
+  d.len = (const double*)up(h.len.data(), sizeof(double) * h.len.size());
+  // include/includes.hpp:53-63 with freq as a runtime value
+  const double PI = 4.0 * std::atan(1.0);
+  const double eps = 8.854187817e-12, c0 = 3.0e8, mu = 4 * PI * 1e-7;
+  d.eps = eps;
+  d.mu = mu;
+  d.omega = 2.0 * PI * h.freq;
+  d.kappa = 2 * PI / (c0 / h.freq);
+  d.C0 = 1.0 / (4 * PI * eps);
+}
+void DeviceGeometry::release() {
+  for (void* p : bufs) (void)hipFree(p);
+  bufs.clear();
+}
+
+namespace {
+
+template <typename T>
+struct Box {
+  bool ILActive = true;
+  double maxPivot_L2P = 0.0, maxPivot_P2M = 0.0;
+  double cx = 0, cy = 0, cz = 0;
+  std::vector<int> IL, NB;
+  bool Eliminated = false;
+  int NumMultipoles = 0, NumLocals = 0;
+  std::map<int, DMat<T>> M2L, P2P, M2P, P2L, L2P_f, P2M_f, M2M_f;
+  DMat<T> L2M_f, L2P, P2M, Ac, Ar;
+  std::vector<int> charge, in_charge, in_check, out_charge, out_check;
+  DMat<T> luP2P, luL2M;
+  int* ipP2P = nullptr;
+  int* ipL2M = nullptr;
+  // solve vectors (n x nrhs)
+  DMat<T> particles, multipoles, locals, particle_rhs, multipole_rhs, local_rhs;
+};
+
+template <typename T>
+class Solver : public SolverBase {
+ public:
+  using R = decltype(T().re);
+  Solver(std::shared_ptr<DeviceGeometry> g, const SolverParams& p, Ctx& c) : G(g), P(p), cx(c) {}
+  ~Solver() override { cx.sync(); }
+
+  // ------------------------------------------------------------ block helpers
+  DMat<T> newmat(int r, int c) {
+    DMat<T> m;
+    m.r = r;
+    m.c = c;
+    m.ld = std::max(r, 1);
+    if ((size_t)r * c > 0) m.p = (T*)cx.pool.alloc(sizeof(T) * (size_t)r * c);
+    return m;
+  }
+  void drop(DMat<T>& m) {
+    if (m.p) cx.pool.free(m.p);
+    m = DMat<T>();
+  }
+  void put(DMat<T>& dst, DMat<T> src) {
+    if (dst.p && dst.p != src.p) cx.pool.free(dst.p);
+    dst = src;
+  }
+  DMat<T> zeros(int r, int c) {
+    DMat<T> m = newmat(r, c);
+    zero2d(m, cx.st);
+    return m;
+  }
+  DMat<T> clone(const DMat<T>& a) {
+    DMat<T> m = newmat(a.r, a.c);
+    copy2d(m, a, cx.st);
+    return m;
+  }
+  struct Group {
+    std::vector<GemmOp<T>> ops;
+  };
+  // C = alpha*op(A)*op(B) + beta*C
+  void add(Group& g, DMat<T> C, DMat<T> A, int opA, DMat<T> B, int opB, T alpha, T beta) {
+    int m = opA == OP_N ? A.r : A.c, k = opA == OP_N ? A.c : A.r;
+    int kb = opB == OP_N ? B.r : B.c, n = opB == OP_N ? B.c : B.r;
+    if (k != kb || C.r != m || C.c != n) throw std::runtime_error("gemm: shape mismatch");
+    if ((size_t)m * n == 0) return;
+    stats_.gemm_flops += 8.0 * m * (double)n * k;
+    if (k == 0) {  // empty inner dimension: C = beta*C
+      if (beta.re == R(0) && beta.im == R(0)) zero2d(C, cx.st);
+      return;
+    }
+    GemmOp<T> o{};
+    o.A = A.p; o.lda = A.ld; o.opA = opA;
+    o.B = B.p; o.ldb = B.ld; o.opB = opB;
+    o.C = C.p; o.ldc = C.ld;
+    o.m = m; o.n = n; o.k = k;
+    o.alpha = alpha; o.beta = beta;
+    g.ops.push_back(o);
+  }
+  void run(Group& g) {
+    gemm_grouped(g.ops, cx);
+    g.ops.clear();
+  }
+  DMat<T> mul(DMat<T> A, int opA, DMat<T> B, int opB, T alpha = T(1, 0)) {
+    int m = opA == OP_N ? A.r : A.c, n = opB == OP_N ? B.c : B.r;
+    DMat<T> C = newmat(m, n);
+    Group g;
+    add(g, C, A, opA, B, opB, alpha, T(0, 0));
+    run(g);
+    return C;
+  }
+  // X = X - prod into an existing block, or a new block -prod when X is empty (T:3936-3947)
+  void assign_or_sub(Group& g, DMat<T>& X, DMat<T> A, int opA, DMat<T> B, int opB) {
+    int m = opA == OP_N ? A.r : A.c, n = opB == OP_N ? B.c : B.r;
+    if (X.size() == 0) {
+      put(X, newmat(m, n));
+      add(g, X, A, opA, B, opB, T(-1, 0), T(0, 0));
+    } else {
+      add(g, X, A, opA, B, opB, T(-1, 0), T(1, 0));
+    }
+  }
+  void factor(DMat<T> A, DMat<T>& lu, int*& ip) {
+    put(lu, clone(A));
+    if (ip) cx.pool.free(ip);
+    ip = (int*)cx.pool.alloc(sizeof(int) * std::max(A.r, 1));
+    double t0 = wall();
+    getrf(lu, ip, cx);
+    stats_.t_getrf += wall() - t0;
+    stats_.getrf_flops += 8.0 / 3.0 * A.r * (double)A.r * A.r;
+  }
+  void solve_inplace(DMat<T> lu, const int* ip, DMat<T> B) {
+    stats_.getrs_flops += 8.0 * lu.r * (double)lu.r * B.c;
+    getrs(lu, ip, B, cx);
+  }
+  std::vector<double> norms(const std::vector<DMat<T>>& v) {
+    std::vector<double> out(v.size(), 0.0);
+    if (v.empty()) return out;
+    sqnorms(v, cx.d_scalars, cx);
+    const double* h = cx.fetch_scalars(v.size());
+    for (size_t i = 0; i < v.size(); ++i) out[i] = std::sqrt(h[i]);
+    return out;
+  }
+  // Index lists travel through the pinned staging ring (stream ordered, no allocation).
+  int* upload_ints(const std::vector<int>& v) {
+    if (v.empty()) return nullptr;
+    return (int*)cx.push(v.data(), sizeof(int) * v.size());
+  }
+  void free_tmp_ints() {}
+  // Blocks Z(rows_i, cols_i) for many pairs in one launch.
+  void entries(const std::vector<const std::vector<int>*>& rows, const std::vector<const std::vector<int>*>& cols,
+               std::vector<DMat<T>*>& outs) {
+    std::vector<EntryTask<T>> tasks;
+    int tiles = 0;
+    for (size_t i = 0; i < rows.size(); ++i) {
+      int nr = (int)rows[i]->size(), nc = (int)cols[i]->size();
+      put(*outs[i], newmat(nr, nc));
+      if (nr == 0 || nc == 0) continue;
+      EntryTask<T> t{};
+      t.rows = upload_ints(*rows[i]);
+      t.cols = upload_ints(*cols[i]);
+      t.nr = nr;
+      t.nc = nc;
+      t.out = outs[i]->p;
+      t.ld = outs[i]->ld;
+      t.tile0 = tiles;
+      tiles += entry_tiles(nr, nc);
+      tasks.push_back(t);
+      // keep one launch under the staging ring and int limits
+      if (tiles > (1 << 28)) throw std::runtime_error("entries: launch too large");
+    }
+    if (!tasks.empty()) {
+      const EntryTask<T>* d = (const EntryTask<T>*)cx.push(tasks.data(), sizeof(EntryTask<T>) * tasks.size());
+      launch_entries<T>(G->d, G->h.kind, d, (int)tasks.size(), tiles, cx.st);
+    }
+  }
+
+  // ------------------------------------------------------------ tree set-up (T:52-331, 425-668)
+  void setup() {
+    const HostGeometry& h = G->h;
+    N = h.n_unknowns();
+    std::vector<double> loc;
+    h.locations(loc);
+    px.resize(N); py.resize(N); pz.resize(N);
+    for (int i = 0; i < N; ++i) { px[i] = loc[3 * i]; py[i] = loc[3 * i + 1]; pz[i] = loc[3 * i + 2]; }
+    TOL_POW = P.tol_pow;
+    fill_in_TOL_POW = P.fillin_tol;
+    fillInTolerance = std::pow(10.0, -1.0 * TOL_POW);
+    const double* bb = P.bbox;
+    L = std::max({bb[1] - bb[0], bb[3] - bb[2], bb[5] - bb[4]});
+    int q = N / P.leafSize;
+    int lv = q > 0 ? (int)std::floor(std::log((double)q) / std::log(8.0)) : 0;
+    if (P.variant == 0) { nLevels = std::max(2, lv); eta = std::sqrt(3.0) * 2; }
+    else { nLevels = std::max(3, lv); eta = std::sqrt(3.0); }
+    nBoxes.assign(1, 1);
+    side.assign(1, L);
+    for (int j = 1; j <= nLevels; ++j) { nBoxes.push_back(8 * nBoxes[j - 1]); side.push_back(0.5 * side[j - 1]); }
+    for (int j = 0; j < 32; ++j) rankVec[j] = 0;
+    int temp = 5, i = 0;
+    for (int j = nLevels; j >= 2; --j) {
+      rankVec[j] = temp;
+      ++i;
+      if (i % 3 == 0) temp += 5;
+    }
+    if (P.variant == 1)
+      for (int j = 0; j < 32; ++j) rankVec[j] = 1 << 28;
+    tree.assign(nLevels + 1, std::vector<Box<T>>());
+    for (int j = 0; j <= nLevels; ++j) tree[j].resize(nBoxes[j]);
+    tree[0][0].cx = (bb[0] + bb[1]) / 2.0;
+    tree[0][0].cy = (bb[2] + bb[3]) / 2.0;
+    tree[0][0].cz = (bb[4] + bb[5]) / 2.0;
+    static const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                     sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int j = 0; j < nLevels; ++j) {
+      double sh = 0.5 * side[j + 1];
+      for (int k = 0; k < nBoxes[j]; ++k)
+        for (int l = 0; l < 8; ++l) {
+          Box<T>& c = tree[j + 1][8 * k + l];
+          c.cx = tree[j][k].cx + sx[l] * sh;
+          c.cy = tree[j][k].cy + sy[l] * sh;
+          c.cz = tree[j][k].cz + sz[l] * sh;
+        }
+    }
+    tree[0][0].NB.push_back(0);
+    for (int j = 0; j < nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k)
+        for (int ci = 0; ci < 8; ++ci) {
+          int child = 8 * k + ci;
+          Box<T>& cb = tree[j + 1][child];
+          for (int nn : tree[j][k].NB)
+            for (int c = 0; c < 8; ++c) {
+              int other = nn * 8 + c;
+              const Box<T>& ob = tree[j + 1][other];
+              double dx = cb.cx - ob.cx, dy = cb.cy - ob.cy, dz = cb.cz - ob.cz;
+              double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+              double etaL = (j + 1 == nLevels) ? eta / 2.0 : eta;  // T:281-284
+              if (dist > etaL * side[j + 1] + side[nLevels] / 2.0) cb.IL.push_back(other);
+              else cb.NB.push_back(other);
+            }
+        }
+    startLevel = -1;
+    for (int j = 0; j <= nLevels && startLevel < 0; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k)
+        if (!tree[j][k].IL.empty()) { startLevel = j; break; }
+    if (startLevel < 0) startLevel = nLevels;
+    // T:576-625: "<=" split against the box centre
+    for (int p = 0; p < N; ++p) tree[0][0].charge.push_back(p);
+    for (int j = 0; j < nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        const Box<T>& b = tree[j][k];
+        for (int idx : b.charge) {
+          int ch;
+          if (pz[idx] <= b.cz) {
+            if (px[idx] <= b.cx) ch = (py[idx] <= b.cy) ? 0 : 3;
+            else ch = (py[idx] <= b.cy) ? 1 : 2;
+          } else {
+            if (px[idx] <= b.cx) ch = (py[idx] <= b.cy) ? 4 : 7;
+            else ch = (py[idx] <= b.cy) ? 5 : 6;
+          }
+          tree[j + 1][8 * k + ch].charge.push_back(idx);
+        }
+      }
+    for (int j = nLevels - 1; j >= 0; --j)
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        auto& cl = tree[j][k].charge;
+        cl.clear();
+        for (int c = 0; c < 8; ++c) {
+          auto& s = tree[j + 1][8 * k + c].charge;
+          cl.insert(cl.end(), s.begin(), s.end());
+        }
+      }
+    stats_.nLevels = nLevels;
+    stats_.startLevel = startLevel;
+  }
+
+  // ------------------------------------------------------------ skeletons (T:681-760, 917-1256)
+  void childPoints(int j, int k, bool inCheck, std::vector<int>& out) {
+    if (j == nLevels) {
+      out.insert(out.end(), tree[j][k].charge.begin(), tree[j][k].charge.end());
+      return;
+    }
+    for (int c = 0; c < 8; ++c) {
+      Box<T>& ch = tree[j + 1][8 * k + c];
+      auto& v = inCheck ? ch.in_check : ch.out_charge;
+      out.insert(out.end(), v.begin(), v.end());
+    }
+  }
+  void getNodes() {
+    const double tol = std::pow(10.0, -TOL_POW);
+    for (int j = nLevels; j >= startLevel; --j) {
+      const int nb = nBoxes[j];
+      // incoming (T:1155-1256)
+      {
+        std::vector<AcaProblem> probs(nb);
+        std::vector<int> which;
+        for (int k = 0; k < nb; ++k) {
+          Box<T>& b = tree[j][k];
+          childPoints(j, k, true, probs[k].rows);
+          for (int kIL : b.IL) childPoints(j, kIL, false, probs[k].cols);
+          if (!probs[k].rows.empty() && !probs[k].cols.empty()) which.push_back(k);
+          else if (probs[k].cols.empty()) { b.ILActive = false; b.in_check = probs[k].rows; }
+          if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
+        }
+        std::vector<AcaResult> res;
+        aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
+        std::vector<const std::vector<int>*> rr, cc;
+        std::vector<DMat<T>*> outs;
+        std::vector<std::vector<int>> pivcols(nb);
+        for (int k : which) {
+          Box<T>& b = tree[j][k];
+          for (int r : res[k].row_bases) b.in_check.push_back(probs[k].rows[r]);
+          for (int c : res[k].col_bases) { b.in_charge.push_back(probs[k].cols[c]); pivcols[k].push_back(probs[k].cols[c]); }
+          rr.push_back(&probs[k].rows);
+          cc.push_back(&pivcols[k]);
+          outs.push_back(&b.Ac);  // Ac = K(rows, pivot columns)
+        }
+        entries(rr, cc, outs);
+      }
+      // outgoing (T:994-1077)
+      {
+        std::vector<AcaProblem> probs(nb);
+        std::vector<int> which;
+        for (int k = 0; k < nb; ++k) {
+          Box<T>& b = tree[j][k];
+          childPoints(j, k, false, probs[k].cols);
+          for (int kIL : b.IL) childPoints(j, kIL, true, probs[k].rows);
+          if (!probs[k].rows.empty() && !probs[k].cols.empty()) which.push_back(k);
+          else if (probs[k].rows.empty()) { b.ILActive = false; b.out_charge = probs[k].cols; }
+          if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
+        }
+        std::vector<AcaResult> res;
+        aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
+        std::vector<const std::vector<int>*> rr, cc;
+        std::vector<DMat<T>*> outs;
+        std::vector<std::vector<int>> pivrows(nb);
+        for (int k : which) {
+          Box<T>& b = tree[j][k];
+          for (int r : res[k].row_bases) { b.out_check.push_back(probs[k].rows[r]); pivrows[k].push_back(probs[k].rows[r]); }
+          for (int c : res[k].col_bases) b.out_charge.push_back(probs[k].cols[c]);
+          rr.push_back(&pivrows[k]);
+          cc.push_back(&probs[k].cols);
+          outs.push_back(&b.Ar);  // Ar = K(pivot rows, cols)
+        }
+        entries(rr, cc, outs);
+      }
+      get_L2P_P2M_level(j);
+    }
+  }
+  void get_L2P_P2M_level(int j) {  // T:681-732, 755-760
+    const int nb = nBoxes[j];
+    std::vector<const std::vector<int>*> rr, cc;
+    std::vector<DMat<T>*> outs;
+    std::vector<DMat<T>> D1(nb), D2(nb);
+    for (int k = 0; k < nb; ++k) {
+      Box<T>& b = tree[j][k];
+      size_t nin = b.in_check.size(), nout = b.out_charge.size();
+      if (nin < nout) {
+        b.out_check.resize(std::min(b.out_check.size(), nin));
+        b.out_charge.resize(nin);
+      } else {
+        b.in_check.resize(nout);
+        if (b.in_charge.size() > nout) b.in_charge.resize(nout);
+      }
+      if (b.ILActive) {
+        if (!b.in_check.empty()) { rr.push_back(&b.in_check); cc.push_back(&b.in_charge); outs.push_back(&D1[k]); }
+        if (!b.out_check.empty()) { rr.push_back(&b.out_check); cc.push_back(&b.out_charge); outs.push_back(&D2[k]); }
+      }
+    }
+    entries(rr, cc, outs);
+    std::vector<DMat<T>> acT, arT;
+    std::vector<int> acK, arK;
+    for (int k = 0; k < nb; ++k) {
+      Box<T>& b = tree[j][k];
+      const int r = (int)b.in_check.size();
+      if (b.ILActive) {
+        DMat<T> temp1 = b.Ac.cols(0, std::min(r, b.Ac.c));
+        if (r > 0) {
+          // L2P = temp1 * D1^{-1} through the LU of D1^H (T:694-696)
+          DMat<T> D1h = newmat(r, r);
+          adjoint2d(D1h, D1[k], cx.st);
+          DMat<T> lu;
+          int* ip = nullptr;
+          factor(D1h, lu, ip);
+          DMat<T> X = newmat(temp1.c, temp1.r);
+          adjoint2d(X, temp1, cx.st);
+          solve_inplace(lu, ip, X);
+          DMat<T> L2P = newmat(X.c, X.r);
+          adjoint2d(L2P, X, cx.st);
+          put(b.L2P, L2P);
+          drop(X); drop(lu); drop(D1h);
+          cx.pool.free(ip);
+        } else {
+          put(b.L2P, zeros(b.Ac.r, r));
+        }
+        const int ro = (int)b.out_check.size();
+        DMat<T> temp2 = b.Ar.rows(0, std::min(ro, b.Ar.r));
+        if (ro > 0) {
+          DMat<T> lu;
+          int* ip = nullptr;
+          factor(D2[k], lu, ip);
+          DMat<T> X = newmat(temp2.r, temp2.c);
+          copy2d(X, temp2, cx.st);
+          solve_inplace(lu, ip, X);
+          put(b.P2M, X);
+          drop(lu);
+          cx.pool.free(ip);
+        } else {
+          put(b.P2M, zeros(ro, b.Ar.c));
+        }
+      } else {
+        DMat<T> I1 = newmat((int)b.in_check.size(), (int)b.in_check.size());
+        fill_identity(I1, T(1, 0), cx.st);
+        put(b.L2P, I1);
+        DMat<T> I2 = newmat((int)b.out_charge.size(), (int)b.out_charge.size());
+        fill_identity(I2, T(1, 0), cx.st);
+        put(b.P2M, I2);
+      }
+      if (b.Ac.size() > 0) { acT.push_back(b.Ac.cols(0, std::min(r, b.Ac.c))); acK.push_back(k); }
+      if (b.Ar.size() > 0) {
+        const int ro = (int)b.out_check.size();
+        DMat<T> t2 = b.Ar.rows(0, std::min(ro, b.Ar.r));
+        DMat<T> adj = newmat(t2.c, t2.r);
+        adjoint2d(adj, t2, cx.st);
+        arT.push_back(adj);
+        arK.push_back(k);
+      }
+      drop(D1[k]);
+      drop(D2[k]);
+    }
+    std::vector<double> mp;
+    max_col_norms(acT, mp, cx);
+    for (size_t i = 0; i < acK.size(); ++i) tree[j][acK[i]].maxPivot_L2P = mp[i];
+    max_col_norms(arT, mp, cx);
+    for (size_t i = 0; i < arK.size(); ++i) tree[j][arK[i]].maxPivot_P2M = mp[i];
+    for (auto& a : arT) drop(a);
+  }
+  void assemble_M2L() {  // T:1258-1273
+    std::vector<const std::vector<int>*> rr, cc;
+    std::vector<DMat<T>*> outs;
+    for (int j = startLevel; j <= nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k)
+        for (int kIL : tree[j][k].IL) {
+          rr.push_back(&tree[j][k].in_check);
+          cc.push_back(&tree[j][kIL].out_charge);
+          outs.push_back(&tree[j][k].M2L[kIL]);
+        }
+    entries(rr, cc, outs);
+  }
+  void free_points() {  // T:1275-1284
+    for (int j = startLevel; j <= nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        Box<T>& b = tree[j][k];
+        b.out_charge.clear();
+        b.in_check.clear();
+        drop(b.Ar);
+        drop(b.Ac);
+      }
+  }
+  void initialise_phase() {  // T:3354-3370
+    for (int j = startLevel; j <= nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        Box<T>& b = tree[j][k];
+        b.NumMultipoles = b.P2M.r;
+        b.NumLocals = b.L2P.c;
+      }
+  }
+  void initialise_P2P_Leaf_Level() {  // T:3372-3386
+    std::vector<const std::vector<int>*> rr, cc;
+    std::vector<DMat<T>*> outs;
+    const int j = nLevels;
+    for (int k = 0; k < nBoxes[j]; ++k) {
+      for (int nn : tree[j][k].NB) {
+        if (nn == k) continue;
+        rr.push_back(&tree[j][k].charge);
+        cc.push_back(&tree[j][nn].charge);
+        outs.push_back(&tree[j][k].P2P[nn]);
+      }
+      rr.push_back(&tree[j][k].charge);
+      cc.push_back(&tree[j][k].charge);
+      outs.push_back(&tree[j][k].P2P[k]);
+    }
+    entries(rr, cc, outs);
+  }
+
+  void assemble() override {
+    double t0 = wall();
+    setup();
+    getNodes();
+    assemble_M2L();
+    free_points();
+    initialise_phase();
+    initialise_P2P_Leaf_Level();
+    cx.sync();
+    stats_.t_assemble = wall() - t0;
+  }
+
+  // ------------------------------------------------------------ elimination
+  bool well_separated(int j, int k, int l) {  // T:4044-4055
+    if (k == l) return false;
+    for (int nn : tree[j][k].NB)
+      if (nn == l) return false;
+    return true;
+  }
+  void initialise_P2P_NonLeafLevel(int j) {  // T:3413-3476
+    std::vector<CopyOp<T>> cops;
+    auto blk = [&](DMat<T>& dst, int r0, int c0, const DMat<T>& src) {
+      if (src.size() == 0) return;
+      if (r0 + src.r > dst.r || c0 + src.c > dst.c) throw std::runtime_error("P2P non-leaf block out of range");
+      CopyOp<T> o{};
+      o.src = src.p; o.lds = src.ld; o.dst = dst.p + r0 + (size_t)c0 * dst.ld; o.ldd = dst.ld;
+      o.r = src.r; o.c = src.c; o.adj = 0;
+      cops.push_back(o);
+    };
+    for (int k = 0; k < nBoxes[j]; ++k) {
+      for (int nn : tree[j][k].NB) {
+        if (nn == k) continue;
+        int n_rows = 0, n_cols = 0;
+        for (int a = 0; a < 8; ++a) {
+          int t = 0;
+          for (int b = 0; b < 8; ++b) t += tree[j + 1][8 * k + a].M2L[8 * nn + b].c;
+          n_cols = std::max(n_cols, t);
+        }
+        for (int b = 0; b < 8; ++b) {
+          int t = 0;
+          for (int a = 0; a < 8; ++a) t += tree[j + 1][8 * k + a].M2L[8 * nn + b].r;
+          n_rows = std::max(n_rows, t);
+        }
+        DMat<T> Pm = zeros(n_rows, n_cols);
+        int row = 0, col = 0;
+        for (int a = 0; a < 8; ++a) {
+          for (int b = 0; b < 8; ++b) {
+            DMat<T>& M = tree[j + 1][8 * k + a].M2L[8 * nn + b];
+            if (M.size() > 0) {
+              blk(Pm, row, col, M);
+              col += M.c;
+            }
+          }
+          row += tree[j + 1][8 * k + a].M2L[8 * nn + 0].r;
+          col = 0;
+        }
+        put(tree[j][k].P2P[nn], Pm);
+      }
+      int n_rows = 0, n_cols = 0;
+      for (int a = 0; a < 8; ++a) n_rows += tree[j + 1][8 * k + a].NumLocals;
+      for (int b = 0; b < 8; ++b) n_cols += tree[j + 1][8 * k + b].NumMultipoles;
+      DMat<T> Pm = zeros(n_rows, n_cols);
+      int row = 0, col = 0;
+      for (int a = 0; a < 8; ++a) {
+        Box<T>& ca = tree[j + 1][8 * k + a];
+        for (int b = 0; b < 8; ++b) {
+          Box<T>& cb = tree[j + 1][8 * k + b];
+          DMat<T>& M = ca.M2L[8 * k + b];
+          if (M.r == ca.NumLocals && M.c == cb.NumMultipoles) blk(Pm, row, col, M);
+          else if (!(M.size() == 0 && (size_t)ca.NumLocals * cb.NumMultipoles == 0))
+            throw std::runtime_error("P2P self block size mismatch");
+          col += cb.NumMultipoles;
+        }
+        row += ca.NumLocals;
+        col = 0;
+      }
+      put(tree[j][k].P2P[k], Pm);
+    }
+    copy_batched(cops, cx);
+  }
+  void deleteM2L(int j) {  // T:3478-3504
+    for (int k = 0; k < nBoxes[j]; ++k)
+      for (int nn : tree[j][k].NB)
+        for (int a = 0; a < 8; ++a)
+          for (int b = 0; b < 8; ++b) drop(tree[j + 1][8 * k + a].M2L[8 * nn + b]);
+  }
+
+  // Concatenate blocks column-wise into a new matrix with `rows` rows.
+  DMat<T> hcat(int rows, const std::vector<DMat<T>>& parts, std::vector<int>& offs) {
+    int total = 0;
+    offs.clear();
+    for (auto& p : parts) {
+      if (p.r != rows && p.size() > 0) throw std::runtime_error("hcat: row mismatch");
+      offs.push_back(total);
+      total += p.c;
+    }
+    DMat<T> X = newmat(rows, total);
+    std::vector<CopyOp<T>> cops;
+    for (size_t i = 0; i < parts.size(); ++i) {
+      if (parts[i].size() == 0) continue;
+      CopyOp<T> o{};
+      o.src = parts[i].p; o.lds = parts[i].ld; o.dst = X.p + (size_t)offs[i] * X.ld; o.ldd = X.ld;
+      o.r = rows; o.c = parts[i].c; o.adj = 0;
+      cops.push_back(o);
+    }
+    copy_batched(cops, cx);
+    return X;
+  }
+
+  void eliminate_x(int j, int k) {  // T:3561-3611, 3653-3791, 3813-4042
+    Box<T>& bk = tree[j][k];
+    DMat<T>& Pkk = bk.P2P[k];
+    if (Pkk.size() != 0) {
+      factor(Pkk, bk.luP2P, bk.ipP2P);
+      std::vector<DMat<T>> parts{bk.L2P};
+      std::vector<int> nbs;
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        nbs.push_back(nn);
+        parts.push_back(tree[j][nn].Eliminated ? bk.M2P[nn] : bk.P2P[nn]);
+      }
+      std::vector<int> offs;
+      DMat<T> X = hcat(Pkk.r, parts, offs);
+      solve_inplace(bk.luP2P, bk.ipP2P, X);
+      DMat<T> XL = X.cols(offs[0], parts[0].c);
+      std::map<int, DMat<T>> Xn;
+      for (size_t i = 0; i < nbs.size(); ++i) Xn[nbs[i]] = X.cols(offs[i + 1], parts[i + 1].c);
+      double t0 = wall();
+      Group g;
+      put(bk.L2M_f, newmat(bk.P2M.r, XL.c));
+      add(g, bk.L2M_f, bk.P2M, OP_N, XL, OP_N, T(-1, 0), T(0, 0));
+      for (int nn : nbs) {
+        DMat<T>& tgt = tree[j][nn].Eliminated ? bk.M2M_f[nn] : bk.P2M_f[nn];
+        put(tgt, newmat(bk.P2M.r, Xn[nn].c));
+        add(g, tgt, bk.P2M, OP_N, Xn[nn], OP_N, T(-1, 0), T(0, 0));
+      }
+      for (int nn : nbs) {
+        Box<T>& bn = tree[j][nn];
+        DMat<T> Ank = bn.Eliminated ? bn.P2L[k] : bn.P2P[k];
+        put(bn.L2P_f[k], newmat(Ank.r, XL.c));
+        add(g, bn.L2P_f[k], Ank, OP_N, XL, OP_N, T(-1, 0), T(0, 0));
+        for (int pp : nbs) {
+          const bool pe = tree[j][pp].Eliminated;
+          DMat<T>& C = !bn.Eliminated ? (!pe ? bn.P2P[pp] : bn.M2P[pp]) : (!pe ? bn.P2L[pp] : bn.M2L[pp]);
+          assign_or_sub(g, C, Ank, OP_N, Xn[pp], OP_N);
+        }
+      }
+      run(g);
+      stats_.t_schur += wall() - t0;
+      drop(X);
+    } else {  // T:3656-3791: zero blocks of the right shapes
+      put(bk.L2M_f, zeros(bk.P2M.r, bk.L2P.c));
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        if (!tree[j][nn].Eliminated) put(bk.P2M_f[nn], zeros(bk.P2M.r, bk.P2P[nn].c));
+        else put(bk.M2M_f[nn], zeros(bk.P2M.r, bk.M2P[nn].c));
+      }
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        Box<T>& bn = tree[j][nn];
+        if (!bn.Eliminated) put(bn.L2P_f[k], zeros(bn.P2P[k].r, bk.L2P.c));
+        else put(bn.L2P_f[k], zeros(bn.P2L[k].r, bk.L2P.c));
+        for (int pp : bk.NB) {
+          if (pp == k) continue;
+          const bool pe = tree[j][pp].Eliminated;
+          if (!bn.Eliminated) {
+            if (!pe) { if (bn.P2P[pp].size() == 0) put(bn.P2P[pp], zeros(bn.P2P[k].r, bk.P2P[pp].c)); }
+            else { if (bn.M2P[pp].size() == 0) put(bn.M2P[pp], zeros(bn.P2P[k].r, bk.M2P[pp].c)); }
+          } else {
+            if (!pe) { if (bn.P2L[pp].size() == 0) put(bn.P2L[pp], zeros(bn.P2L[k].r, bk.P2P[pp].c)); }
+            else (void)bn.M2L[pp];
+          }
+        }
+      }
+    }
+  }
+
+  void eliminate_z(int j, int k) {  // T:4081-4113, 4150-4509
+    Box<T>& bk = tree[j][k];
+    if (bk.L2M_f.size() > 0) {
+      factor(bk.L2M_f, bk.luL2M, bk.ipL2M);
+      const int NM = bk.NumMultipoles;
+      DMat<T> mI = newmat(NM, NM);
+      fill_identity(mI, T(-1, 0), cx.st);
+      std::vector<DMat<T>> parts{mI};
+      std::vector<int> nbs;
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        nbs.push_back(nn);
+        parts.push_back(tree[j][nn].Eliminated ? bk.M2M_f[nn] : bk.P2M_f[nn]);
+      }
+      std::vector<int> offs;
+      DMat<T> Y = hcat(bk.L2M_f.r, parts, offs);
+      drop(mI);
+      solve_inplace(bk.luL2M, bk.ipL2M, Y);
+      DMat<T> Yself = Y.cols(offs[0], NM);
+      put(bk.M2L[k], clone(Yself));
+      std::map<int, DMat<T>> Yn;
+      for (size_t i = 0; i < nbs.size(); ++i) {
+        int nn = nbs[i];
+        Yn[nn] = Y.cols(offs[i + 1], parts[i + 1].c);
+        DMat<T>& tgt = tree[j][nn].Eliminated ? bk.M2L[nn] : bk.P2L[nn];
+        put(tgt, clone(Yn[nn]));
+      }
+      double t0 = wall();
+      Group g;
+      for (int nn : nbs) {
+        Box<T>& bn = tree[j][nn];
+        DMat<T> Lf = bn.L2P_f[k];
+        DMat<T>& ty = !bn.Eliminated ? bn.M2P[k] : bn.M2L[k];
+        assign_or_sub(g, ty, Lf, OP_N, Yself, OP_N);
+        for (int pp : nbs) {
+          const bool pe = tree[j][pp].Eliminated;
+          DMat<T>& C = !bn.Eliminated ? (!pe ? bn.P2P[pp] : bn.M2P[pp]) : (!pe ? bn.P2L[pp] : bn.M2L[pp]);
+          if (C.size() == 0 && (size_t)Lf.r * Yn[pp].c > 0) throw std::runtime_error("eliminate_z: missing fill-in block");
+          add(g, C, Lf, OP_N, Yn[pp], OP_N, T(-1, 0), T(1, 0));
+        }
+      }
+      run(g);
+      stats_.t_schur += wall() - t0;
+      drop(Y);
+      // fill-in pairs to recompress (T:4413-4418, 4445-4453), pushed in loop order
+      std::vector<DMat<T>> nm;
+      std::vector<std::array<int, 3>> cand;  // (type, nn, pp)
+      for (int nn : nbs) {
+        Box<T>& bn = tree[j][nn];
+        for (int pp : nbs) {
+          const bool pe = tree[j][pp].Eliminated;
+          if (!bn.Eliminated && !pe && well_separated(j, nn, pp) && nn < pp) {
+            cand.push_back({0, nn, pp});
+            nm.push_back(bn.P2P[pp]);
+            nm.push_back(bn.M2L[pp]);
+          } else if (bn.Eliminated && !pe && well_separated(j, nn, pp)) {
+            cand.push_back({1, nn, pp});
+            nm.push_back(bn.P2L[pp]);
+            nm.push_back(bn.M2L[pp]);
+          }
+        }
+      }
+      std::vector<double> v = norms(nm);
+      for (size_t i = 0; i < cand.size(); ++i) {
+        double f = v[2 * i], m = v[2 * i + 1];
+        if (m > 0 && f / m > fillInTolerance) {
+          if (cand[i][0] == 0) P2Plist.push_back({cand[i][1], cand[i][2]});
+          else P2L_M2P.push_back({cand[i][1], cand[i][2]});
+        }
+      }
+    } else {  // T:4150-4296
+      put(bk.M2L[k], newmat(0, bk.NumMultipoles));
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        if (!tree[j][nn].Eliminated) put(bk.P2L[nn], newmat(0, bk.P2M_f[nn].c));
+        else put(bk.M2L[nn], newmat(0, bk.M2M_f[nn].c));
+      }
+      for (int nn : bk.NB) {
+        if (nn == k) continue;
+        Box<T>& bn = tree[j][nn];
+        if (!bn.Eliminated) {
+          if (bn.M2P[k].size() == 0) put(bn.M2P[k], zeros(bn.L2P_f[k].r, bk.NumMultipoles));
+        } else {
+          if (bn.M2L[k].size() == 0) put(bn.M2L[k], zeros(bn.L2P_f[k].r, bk.NumMultipoles));
+        }
+      }
+    }
+  }
+
+  // ------------------------------------------------------------ fill-in recompression
+  struct QR {
+    DMat<T> Q;
+    double maxPivot;
+  };
+  QR do_rrqr(DMat<T> A, int partition, double inputPivot, int j) {
+    DMat<T> W = clone(A);
+    DMat<T> Qb = newmat(A.r, std::max(1, std::min(A.r, A.c)));
+    double t0 = wall();
+    RRQRInfo inf = rrqr(W, partition, std::pow(10.0, -TOL_POW), std::pow(10.0, -fill_in_TOL_POW), inputPivot,
+                        rankVec[j], Qb, cx);
+    stats_.t_compress += wall() - t0;
+    stats_.n_rrqr++;
+    double r = inf.rank, m = A.r, n = A.c;
+    stats_.qr_flops += 8.0 * (2.0 * m * n * r - r * r * (m + n) + 2.0 * r * r * r / 3.0) + 8.0 * (2.0 * m * r * r - 2.0 * r * r * r / 3.0);
+    QR q;
+    q.Q = newmat(A.r, inf.rank);
+    if (inf.rank > 0) copy2d(q.Q, Qb.cols(0, inf.rank), cx.st);
+    q.maxPivot = inf.maxPivot;
+    drop(W);
+    drop(Qb);
+    return q;
+  }
+  DMat<T> hcat2(DMat<T> A, int opA, DMat<T> B, int opB) {  // [op(A) | op(B)]
+    int ra = opA == OP_N ? A.r : A.c, ca = opA == OP_N ? A.c : A.r;
+    int rb = opB == OP_N ? B.r : B.c, cb = opB == OP_N ? B.c : B.r;
+    if (ra != rb) throw std::runtime_error("hcat2: row mismatch");
+    DMat<T> X = newmat(ra, ca + cb);
+    std::vector<CopyOp<T>> cops(2);
+    cops[0] = CopyOp<T>{A.p, X.p, A.ld, X.ld, ra, ca, opA, 0};
+    cops[1] = CopyOp<T>{B.p, X.p + (size_t)ca * X.ld, B.ld, X.ld, rb, cb, opB, 0};
+    copy_batched(cops, cx);
+    return X;
+  }
+  void update_other_M2Ls_at_k_old(int j, int k, int b, DMat<T> newL2P) {  // T:3159-3188
+    Box<T>& bk = tree[j][k];
+    DMat<T> R = mul(newL2P, OP_C, bk.L2P, OP_N);
+    std::vector<DMat<T>*> tg;
+    for (auto& kv : bk.P2L) tg.push_back(&kv.second);
+    for (int kIL : bk.IL)
+      if (kIL != b) tg.push_back(&bk.M2L[kIL]);
+    for (int kN : bk.NB)
+      if (kN != k && tree[j][kN].Eliminated && bk.Eliminated) tg.push_back(&bk.M2L[kN]);
+    if (bk.Eliminated) tg.push_back(&bk.M2L[k]);
+    Group g;
+    std::vector<DMat<T>> fresh;
+    for (DMat<T>* X : tg) {
+      if (X->r != R.c) throw std::runtime_error("update_other_M2Ls_at_k: shape mismatch");
+      DMat<T> Y = newmat(R.r, X->c);
+      add(g, Y, R, OP_N, *X, OP_N, T(1, 0), T(0, 0));
+      fresh.push_back(Y);
+    }
+    run(g);
+    for (size_t i = 0; i < tg.size(); ++i) put(*tg[i], fresh[i]);
+    drop(R);
+  }
+  void update_other_M2Ls_at_b_old(int j, int k, int b, DMat<T> newP2Mt) {  // T:3252-3288
+    Box<T>& bb = tree[j][b];
+    DMat<T> R = mul(bb.P2M, OP_N, newP2Mt, OP_N);
+    std::vector<DMat<T>*> tg;
+    for (auto& kv : bb.P2L) tg.push_back(&tree[j][kv.first].M2P[b]);
+    for (int kIL : bb.IL)
+      if (kIL != k) tg.push_back(&tree[j][kIL].M2L[b]);
+    for (int kN : bb.NB)
+      if (kN != b && tree[j][kN].Eliminated && bb.Eliminated) tg.push_back(&tree[j][kN].M2L[b]);
+    if (bb.Eliminated) tg.push_back(&bb.M2L[b]);
+    Group g;
+    std::vector<DMat<T>> fresh;
+    for (DMat<T>* X : tg) {
+      if (X->c != R.r) throw std::runtime_error("update_other_M2Ls_at_b: shape mismatch");
+      DMat<T> Y = newmat(X->r, R.c);
+      add(g, Y, *X, OP_N, R, OP_N, T(1, 0), T(0, 0));
+      fresh.push_back(Y);
+    }
+    run(g);
+    for (size_t i = 0; i < tg.size(); ++i) put(*tg[i], fresh[i]);
+    drop(R);
+  }
+  void update_parent_L2P_old(int j, int k, DMat<T> newL2P) {  // T:3290-3314
+    int kP = k / 8, ck = k % 8;
+    Box<T>& par = tree[j - 1][kP];
+    std::vector<int> r0(8), nr(8);
+    int row = 0;
+    for (int c = 0; c < 8; ++c) { r0[c] = row; nr[c] = tree[j][kP * 8 + c].NumLocals; row += nr[c]; }
+    DMat<T> Rm = mul(newL2P, OP_C, tree[j][k].L2P, OP_N);
+    DMat<T> Uk = mul(Rm, OP_N, par.L2P.rows(r0[ck], nr[ck]), OP_N);
+    int total = row - nr[ck] + Uk.r;
+    DMat<T> Pn = newmat(total, par.L2P.c);
+    std::vector<CopyOp<T>> cops;
+    int out = 0;
+    for (int c = 0; c < 8; ++c) {
+      DMat<T> src = (c == ck) ? Uk : par.L2P.rows(r0[c], nr[c]);
+      if (src.size()) cops.push_back(CopyOp<T>{src.p, Pn.p + out, src.ld, Pn.ld, src.r, src.c, 0, 0});
+      out += src.r;
+    }
+    copy_batched(cops, cx);
+    put(par.L2P, Pn);
+    drop(Rm);
+    drop(Uk);
+  }
+  void update_parent_P2M_old(int j, int b, DMat<T> newP2Mt) {  // T:3316-3340
+    int bP = b / 8, cb = b % 8;
+    Box<T>& par = tree[j - 1][bP];
+    std::vector<int> c0(8), nc(8);
+    int col = 0;
+    for (int c = 0; c < 8; ++c) { c0[c] = col; nc[c] = tree[j][bP * 8 + c].NumMultipoles; col += nc[c]; }
+    DMat<T> Rm = mul(tree[j][b].P2M, OP_N, newP2Mt, OP_N);
+    DMat<T> Vb = mul(par.P2M.cols(c0[cb], nc[cb]), OP_N, Rm, OP_N);
+    int total = col - nc[cb] + Vb.c;
+    DMat<T> Pn = newmat(par.P2M.r, total);
+    std::vector<CopyOp<T>> cops;
+    int out = 0;
+    for (int c = 0; c < 8; ++c) {
+      DMat<T> src = (c == cb) ? Vb : par.P2M.cols(c0[c], nc[c]);
+      if (src.size()) cops.push_back(CopyOp<T>{src.p, Pn.p + (size_t)out * Pn.ld, src.ld, Pn.ld, src.r, src.c, 0, 0});
+      out += src.c;
+    }
+    copy_batched(cops, cx);
+    put(par.P2M, Pn);
+    drop(Rm);
+    drop(Vb);
+  }
+  // T:1573-1656
+  void compress_P2P_myRRQR(int j, int k, int b, DMat<T> oldL2P, DMat<T> oldP2M, DMat<T> F, DMat<T>& newL2P,
+                           DMat<T>& newP2Mt, DMat<T>& M2Lkb) {
+    DMat<T> Acol = hcat2(oldL2P, OP_N, F, OP_N);
+    QR q1 = do_rrqr(Acol, oldL2P.c, tree[j][k].maxPivot_L2P, j);
+    drop(Acol);
+    tree[j][k].maxPivot_L2P = std::max(tree[j][k].maxPivot_L2P, q1.maxPivot);
+    newL2P = q1.Q;
+    DMat<T> Rk = mul(newL2P, OP_C, oldL2P, OP_N);
+    DMat<T> Rkp = mul(newL2P, OP_C, F, OP_N);
+    DMat<T> Arow = hcat2(oldP2M, OP_C, Rkp, OP_C);
+    QR q2 = do_rrqr(Arow, oldP2M.r, tree[j][b].maxPivot_P2M, j);
+    drop(Arow);
+    tree[j][b].maxPivot_P2M = std::max(tree[j][b].maxPivot_P2M, q2.maxPivot);
+    newP2Mt = q2.Q;
+    DMat<T> Rn = mul(newP2Mt, OP_C, oldP2M, OP_C);
+    DMat<T> Rnp = mul(newP2Mt, OP_C, Rkp, OP_C);
+    DMat<T>& M = tree[j][k].M2L[b];
+    if (M.size() > 0) {
+      DMat<T> t = mul(Rk, OP_N, M, OP_N);
+      M2Lkb = newmat(Rnp.c, Rnp.r);
+      adjoint2d(M2Lkb, Rnp, cx.st);
+      Group g;
+      add(g, M2Lkb, t, OP_N, Rn, OP_C, T(1, 0), T(1, 0));
+      run(g);
+      drop(t);
+    } else {
+      M2Lkb = newmat(Rnp.c, Rnp.r);
+      adjoint2d(M2Lkb, Rnp, cx.st);
+    }
+    drop(Rk); drop(Rkp); drop(Rn); drop(Rnp);
+  }
+  void square_up(DMat<T> L2P, DMat<T> P2Mt, DMat<T>& L2Pm, DMat<T>& P2Mtm) {  // T:2168-2183
+    if (L2P.c < P2Mt.c) { P2Mtm = P2Mt.cols(0, L2P.c); L2Pm = L2P; }
+    else { L2Pm = L2P.cols(0, P2Mt.c); P2Mtm = P2Mt; }
+  }
+  void compress_P2P_Two_Ways(int j, int k, int b) {  // T:2116-2306
+    if (tree[j][b].M2L[k].size() == 0 || tree[j][k].M2L[b].size() == 0) return;
+    stats_.n_compress_p2p++;
+    DMat<T> nL2Pk, nP2Mtb, M2Lkb, nL2Pb, nP2Mtk, M2Lbk;
+    compress_P2P_myRRQR(j, k, b, tree[j][k].L2P, tree[j][b].P2M, tree[j][k].P2P[b], nL2Pk, nP2Mtb, M2Lkb);
+    compress_P2P_myRRQR(j, b, k, tree[j][b].L2P, tree[j][k].P2M, tree[j][b].P2P[k], nL2Pb, nP2Mtk, M2Lbk);
+    DMat<T> L2Pk_m, P2Mtk_m, L2Pb_m, P2Mtb_m;
+    square_up(nL2Pk, nP2Mtk, L2Pk_m, P2Mtk_m);
+    square_up(nL2Pb, nP2Mtb, L2Pb_m, P2Mtb_m);
+    put(tree[j][k].M2L[b], clone(M2Lkb.rows(0, L2Pk_m.c).cols(0, P2Mtb_m.c)));
+    update_other_M2Ls_at_k_old(j, k, b, L2Pk_m);
+    update_other_M2Ls_at_b_old(j, k, b, P2Mtb_m);
+    if (j > startLevel) {
+      update_parent_L2P_old(j, k, L2Pk_m);
+      update_parent_P2M_old(j, b, P2Mtb_m);
+    }
+    put(tree[j][k].L2P, clone(L2Pk_m));
+    {
+      DMat<T> t = newmat(P2Mtb_m.c, P2Mtb_m.r);
+      adjoint2d(t, P2Mtb_m, cx.st);
+      put(tree[j][b].P2M, t);
+    }
+    tree[j][k].NumLocals = tree[j][k].L2P.c;
+    tree[j][b].NumMultipoles = tree[j][b].P2M.r;
+    put(tree[j][b].M2L[k], clone(M2Lbk.rows(0, L2Pb_m.c).cols(0, P2Mtk_m.c)));
+    update_other_M2Ls_at_k_old(j, b, k, L2Pb_m);
+    update_other_M2Ls_at_b_old(j, b, k, P2Mtk_m);
+    if (j > startLevel) {
+      update_parent_L2P_old(j, b, L2Pb_m);
+      update_parent_P2M_old(j, k, P2Mtk_m);
+    }
+    put(tree[j][b].L2P, clone(L2Pb_m));
+    {
+      DMat<T> t = newmat(P2Mtk_m.c, P2Mtk_m.r);
+      adjoint2d(t, P2Mtk_m, cx.st);
+      put(tree[j][k].P2M, t);
+    }
+    tree[j][b].NumLocals = tree[j][b].L2P.c;
+    tree[j][k].NumMultipoles = tree[j][k].P2M.r;
+    drop(nL2Pk); drop(nP2Mtb); drop(M2Lkb); drop(nL2Pb); drop(nP2Mtk); drop(M2Lbk);
+  }
+  void compress_M2P_P2L(int j, int k, int b) {  // T:2454-2596, 2878-2959 (k not eliminated, b eliminated)
+    if (tree[j][b].M2L[k].size() == 0 || tree[j][k].M2L[b].size() == 0) return;
+    stats_.n_compress_p2l++;
+    Box<T>& bk = tree[j][k];
+    Box<T>& bb = tree[j][b];
+    DMat<T> Acol = hcat2(bk.L2P, OP_N, bk.M2P[b], OP_N);
+    QR q1 = do_rrqr(Acol, bk.L2P.c, bk.maxPivot_L2P, j);
+    drop(Acol);
+    DMat<T> nL2P = q1.Q;
+    bk.maxPivot_L2P = std::max(bk.maxPivot_L2P, q1.maxPivot);
+    DMat<T> Rk = mul(nL2P, OP_C, bk.L2P, OP_N);
+    DMat<T> M2Lkb = mul(nL2P, OP_C, bk.M2P[b], OP_N);  // Rk' then += Rk*M2L
+    {
+      Group g;
+      add(g, M2Lkb, Rk, OP_N, bk.M2L[b], OP_N, T(1, 0), T(1, 0));
+      run(g);
+    }
+    DMat<T> Arow = hcat2(bk.P2M, OP_C, bb.P2L[k], OP_C);
+    QR q2 = do_rrqr(Arow, bk.P2M.r, bk.maxPivot_P2M, j);
+    drop(Arow);
+    DMat<T> nP2Mt = q2.Q;
+    bk.maxPivot_P2M = std::max(bk.maxPivot_P2M, q2.maxPivot);
+    DMat<T> Rn = mul(nP2Mt, OP_C, bk.P2M, OP_C);
+    DMat<T> Rnp = mul(nP2Mt, OP_C, bb.P2L[k], OP_C);
+    DMat<T> M2Lbk = newmat(Rnp.c, Rnp.r);
+    adjoint2d(M2Lbk, Rnp, cx.st);
+    {
+      Group g;
+      add(g, M2Lbk, bb.M2L[k], OP_N, Rn, OP_C, T(1, 0), T(1, 0));
+      run(g);
+    }
+    DMat<T> L2Pm, P2Mtm;
+    square_up(nL2P, nP2Mt, L2Pm, P2Mtm);
+    put(bk.M2L[b], clone(M2Lkb.rows(0, L2Pm.c)));
+    update_other_M2Ls_at_k_old(j, k, b, L2Pm);
+    if (j > startLevel) update_parent_L2P_old(j, k, L2Pm);
+    put(bk.L2P, clone(L2Pm));
+    bk.NumLocals = bk.L2P.c;
+    put(bb.M2L[k], clone(M2Lbk.cols(0, P2Mtm.c)));
+    update_other_M2Ls_at_b_old(j, b, k, P2Mtm);
+    if (j > startLevel) update_parent_P2M_old(j, k, P2Mtm);
+    {
+      DMat<T> t = newmat(P2Mtm.c, P2Mtm.r);
+      adjoint2d(t, P2Mtm, cx.st);
+      put(bk.P2M, t);
+    }
+    bk.NumMultipoles = bk.P2M.r;
+    drop(nL2P); drop(Rk); drop(M2Lkb); drop(nP2Mt); drop(Rn); drop(Rnp); drop(M2Lbk);
+  }
+
+  void factorize() override {  // T:4599-4747
+    double t0 = wall();
+    for (int j = nLevels; j >= startLevel; --j) {
+      if (j != nLevels) {
+        initialise_P2P_NonLeafLevel(j);
+        deleteM2L(j);
+      }
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        double tc = wall();
+        for (size_t i = 0; i < P2Plist.size(); ++i) {
+          int nn = P2Plist[i].first, pp = P2Plist[i].second;
+          if (k == nn || k == pp) {
+            P2Plist.erase(P2Plist.begin() + i);
+            --i;
+            if (tree[j][nn].P2P[pp].size() != 0) {
+              compress_P2P_Two_Ways(j, nn, pp);
+              drop(tree[j][nn].P2P[pp]);
+              drop(tree[j][pp].P2P[nn]);
+            }
+          }
+        }
+        for (size_t i = 0; i < P2L_M2P.size(); ++i) {
+          int nn = P2L_M2P[i].first, pp = P2L_M2P[i].second;
+          if (k == nn || k == pp) {
+            P2L_M2P.erase(P2L_M2P.begin() + i);
+            --i;
+            if (tree[j][nn].P2L[pp].size() != 0) {
+              compress_M2P_P2L(j, pp, nn);
+              drop(tree[j][nn].P2L[pp]);
+              drop(tree[j][pp].M2P[nn]);
+            }
+          }
+        }
+        (void)tc;
+        tree[j][k].Eliminated = true;
+        eliminate_x(j, k);
+        eliminate_z(j, k);
+      }
+    }
+    factor_base_level();
+    cx.sync();
+    stats_.t_factor = wall() - t0;
+    stats_.peak_bytes = cx.pool.peak();
+  }
+  // T:4772-4810: the base-level system is factored once here; the reference builds
+  // and factors it inside every back substitution.
+  void factor_base_level() {
+    const int j = startLevel;
+    NMb = 0;
+    NLb = 0;
+    for (int k = 0; k < nBoxes[j]; ++k) { NMb += tree[j][k].NumMultipoles; NLb += tree[j][k].NumLocals; }
+    if (NLb == 0 || NMb == 0) return;
+    if (NLb != NMb) throw std::runtime_error("base-level system is not square");
+    DMat<T> A = zeros(NLb, NMb);
+    std::vector<CopyOp<T>> cops;
+    int ri = 0;
+    for (int r = 0; r < nBoxes[j]; ++r) {
+      int si = 0;
+      for (int s = 0; s < nBoxes[j]; ++s) {
+        auto it = tree[j][r].M2L.find(s);
+        if (it != tree[j][r].M2L.end() && it->second.r == tree[j][r].NumLocals &&
+            it->second.c == tree[j][s].NumMultipoles && it->second.size() > 0)
+          cops.push_back(CopyOp<T>{it->second.p, A.p + ri + (size_t)si * A.ld, it->second.ld, A.ld, it->second.r,
+                                   it->second.c, 0, 0});
+        si += tree[j][s].NumMultipoles;
+      }
+      ri += tree[j][r].NumLocals;
+    }
+    copy_batched(cops, cx);
+    factor(A, luBase, ipBase);
+    drop(A);
+  }
+
+  // ------------------------------------------------------------ solve
+  DMat<T> vcat_children_mrhs(int j, int k, int nrhs) {
+    int len = 0;
+    for (int c = 0; c < 8; ++c) len += tree[j + 1][8 * k + c].multipole_rhs.r;
+    DMat<T> v = newmat(len, nrhs);
+    std::vector<CopyOp<T>> cops;
+    int off = 0;
+    for (int c = 0; c < 8; ++c) {
+      DMat<T>& m = tree[j + 1][8 * k + c].multipole_rhs;
+      if (m.size()) cops.push_back(CopyOp<T>{m.p, v.p + off, m.ld, v.ld, m.r, nrhs, 0, 0});
+      off += m.r;
+    }
+    copy_batched(cops, cx);
+    return v;
+  }
+  void solve(const double* rhs, int nrhs, double* x, double* x_box) override {
+    double t0 = wall();
+    // assign_Leaf_rhs (T:670-678)
+    DMat<T> B = newmat(N, nrhs);
+    {
+      DMat<cpx<double>> B64;
+      B64.r = N; B64.c = nrhs; B64.ld = N;
+      B64.p = (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs);
+      HIPCHK(hipMemcpyAsync(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs, hipMemcpyHostToDevice, cx.st));
+      to_T(B, B64);
+      cx.pool.free(B64.p);
+    }
+    for (int k = 0; k < nBoxes[nLevels]; ++k) {
+      Box<T>& b = tree[nLevels][k];
+      put(b.particle_rhs, newmat((int)b.charge.size(), nrhs));
+      if (!b.charge.empty()) gather_rows(b.particle_rhs, B, upload_ints(b.charge), cx.st);
+    }
+    double ts = wall();
+    // rhs_eliminate_phase_efficient (T:4749-4770)
+    for (int j = nLevels; j >= startLevel; --j)
+      for (int k = 0; k < nBoxes[j]; ++k) tree[j][k].Eliminated = false;
+    for (int j = nLevels; j >= startLevel; --j) {
+      if (j != nLevels)
+        for (int k = 0; k < nBoxes[j]; ++k) put(tree[j][k].particle_rhs, vcat_children_mrhs(j, k, nrhs));
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        Box<T>& bk = tree[j][k];
+        bk.Eliminated = true;
+        if (bk.P2P[k].size() != 0) {  // T:3613-3653
+          DMat<T> s = clone(bk.particle_rhs);
+          solve_inplace(bk.luP2P, bk.ipP2P, s);
+          Group g;
+          put(bk.local_rhs, newmat(bk.P2M.r, nrhs));
+          add(g, bk.local_rhs, bk.P2M, OP_N, s, OP_N, T(-1, 0), T(0, 0));
+          for (int nn : bk.NB) {
+            if (nn == k) continue;
+            Box<T>& bn = tree[j][nn];
+            if (!bn.Eliminated) add(g, bn.particle_rhs, bn.P2P[k], OP_N, s, OP_N, T(-1, 0), T(1, 0));
+            else add(g, bn.multipole_rhs, bn.P2L[k], OP_N, s, OP_N, T(-1, 0), T(1, 0));
+          }
+          run(g);
+          drop(s);
+        } else {
+          put(bk.local_rhs, zeros(bk.P2M.r, nrhs));
+        }
+        if (bk.L2M_f.size() > 0) {  // T:4115-4148
+          DMat<T> s = clone(bk.local_rhs);
+          solve_inplace(bk.luL2M, bk.ipL2M, s);
+          Group g;
+          for (int nn : bk.NB) {
+            if (nn == k) continue;
+            Box<T>& bn = tree[j][nn];
+            if (!bn.Eliminated) add(g, bn.particle_rhs, bn.L2P_f[k], OP_N, s, OP_N, T(-1, 0), T(1, 0));
+            else add(g, bn.multipole_rhs, bn.L2P_f[k], OP_N, s, OP_N, T(-1, 0), T(1, 0));
+          }
+          run(g);
+          put(bk.multipole_rhs, s);
+        } else {
+          put(bk.multipole_rhs, newmat(0, nrhs));
+        }
+      }
+    }
+    // back_substitution_phase (T:4812-4865)
+    {
+      const int j = startLevel;
+      if (NLb > 0 && NMb > 0) {
+        DMat<T> b = newmat(NLb, nrhs);
+        std::vector<CopyOp<T>> cops;
+        int ri = 0;
+        for (int r = 0; r < nBoxes[j]; ++r) {
+          DMat<T>& m = tree[j][r].multipole_rhs;
+          if (m.size()) cops.push_back(CopyOp<T>{m.p, b.p + ri, m.ld, b.ld, m.r, nrhs, 0, 0});
+          ri += tree[j][r].NumLocals;
+        }
+        copy_batched(cops, cx);
+        solve_inplace(luBase, ipBase, b);
+        int idx = 0;
+        for (int k = 0; k < nBoxes[j]; ++k) {
+          int nm = tree[j][k].NumMultipoles;
+          DMat<T> mm = newmat(nm, nrhs);
+          if (nm) copy2d(mm, b.rows(idx, nm), cx.st);
+          put(tree[j][k].multipoles, mm);
+          idx += nm;
+        }
+        drop(b);
+      }
+    }
+    for (int j = startLevel; j <= nLevels; ++j) {
+      for (int k = nBoxes[j] - 1; k >= 0; --k) {
+        Box<T>& bk = tree[j][k];
+        if (bk.P2P[k].size() != 0) {
+          DMat<T> rhs1 = clone(bk.particle_rhs);
+          Group g;
+          for (int nn : bk.NB) {
+            if (nn == k) continue;
+            Box<T>& bn = tree[j][nn];
+            if (!bn.Eliminated) add(g, rhs1, bk.P2P[nn], OP_N, bn.particles, OP_N, T(-1, 0), T(1, 0));
+            else add(g, rhs1, bk.M2P[nn], OP_N, bn.multipoles, OP_N, T(-1, 0), T(1, 0));
+          }
+          run(g);
+          DMat<T> t = clone(rhs1);
+          solve_inplace(bk.luP2P, bk.ipP2P, t);
+          DMat<T> rhs2 = clone(bk.multipoles);
+          {
+            Group g2;
+            add(g2, rhs2, bk.P2M, OP_N, t, OP_N, T(-1, 0), T(1, 0));
+            run(g2);
+          }
+          if (bk.L2M_f.size() != 0) {  // Atemp == L2M_f
+            solve_inplace(bk.luL2M, bk.ipL2M, rhs2);
+            put(bk.locals, rhs2);
+          } else {
+            drop(rhs2);
+            put(bk.locals, newmat(0, nrhs));
+          }
+          {
+            Group g3;
+            add(g3, rhs1, bk.L2P, OP_N, bk.locals, OP_N, T(-1, 0), T(1, 0));
+            run(g3);
+          }
+          solve_inplace(bk.luP2P, bk.ipP2P, rhs1);
+          put(bk.particles, rhs1);
+          drop(t);
+        } else {
+          put(bk.particles, newmat(0, nrhs));
+        }
+        bk.Eliminated = false;
+        if (j != nLevels) {
+          int idx = 0;
+          for (int c = 0; c < 8; ++c) {
+            Box<T>& ch = tree[j + 1][8 * k + c];
+            DMat<T> mm = newmat(ch.NumMultipoles, nrhs);
+            if (ch.NumMultipoles) copy2d(mm, bk.particles.rows(idx, ch.NumMultipoles), cx.st);
+            put(ch.multipoles, mm);
+            idx += ch.NumMultipoles;
+          }
+        }
+      }
+    }
+    // getx (T:4874-4907) and reorder (T:466-476)
+    DMat<T> Xb = zeros(N, nrhs);
+    std::vector<CopyOp<T>> cops;
+    int off = 0;
+    std::vector<int> order;
+    for (int k = 0; k < nBoxes[nLevels]; ++k) {
+      Box<T>& b = tree[nLevels][k];
+      if (b.particles.size()) cops.push_back(CopyOp<T>{b.particles.p, Xb.p + off, b.particles.ld, Xb.ld, b.particles.r, nrhs, 0, 0});
+      off += (int)b.charge.size();
+      order.insert(order.end(), b.charge.begin(), b.charge.end());
+    }
+    copy_batched(cops, cx);
+    DMat<T> Xo = zeros(N, nrhs);
+    scatter_rows(Xo, Xb, upload_ints(order), cx.st);
+    cx.sync();
+    stats_.t_solve = wall() - ts;
+    download(Xo, x);
+    if (x_box) download(Xb, x_box);
+    free_tmp_ints();
+    drop(Xb); drop(Xo); drop(B);
+    (void)t0;
+  }
+  void to_T(DMat<T> dst, DMat<cpx<double>> src);
+  void download(DMat<T> src, double* host);
+
+  SolverStats stats() const override {
+    SolverStats s = stats_;
+    s.factor_bytes = 0;
+    for (int j = startLevel; j <= nLevels; ++j)
+      for (int k = 0; k < (int)tree[j].size(); ++k) {
+        const Box<T>& b = tree[j][k];
+        auto acc = [&](const DMat<T>& m) { s.factor_bytes += (double)m.size() * sizeof(T); };
+        acc(b.luP2P); acc(b.luL2M); acc(b.P2M); acc(b.L2P);
+        for (auto& kv : b.P2P) if (std::find(b.NB.begin(), b.NB.end(), kv.first) != b.NB.end()) acc(kv.second);
+        for (auto& kv : b.M2P) acc(kv.second);
+        for (auto& kv : b.P2L) acc(kv.second);
+        for (auto& kv : b.L2P_f) acc(kv.second);
+      }
+    s.factor_bytes += (double)luBase.size() * sizeof(T);
+    return s;
+  }
+  void ranks(int level, std::vector<int>& out) override {
+    out.clear();
+    if (level < 0 || level > nLevels) return;
+    for (auto& b : tree[level]) {
+      out.push_back(b.NumMultipoles);
+      out.push_back(b.NumLocals);
+      out.push_back((int)b.charge.size());
+    }
+  }
+
+ private:
+  std::shared_ptr<DeviceGeometry> G;
+  SolverParams P;
+  Ctx& cx;
+  SolverStats stats_;
+  int N = 0, nLevels = 0, startLevel = 0, TOL_POW = 6, fill_in_TOL_POW = 6;
+  double L = 0, eta = 0, fillInTolerance = 0;
+  int rankVec[32];
+  std::vector<int> nBoxes;
+  std::vector<double> side, px, py, pz;
+  std::vector<std::vector<Box<T>>> tree;
+  std::vector<std::pair<int, int>> P2Plist, P2L_M2P;
+  DMat<T> luBase;
+  int* ipBase = nullptr;
+  int NMb = 0, NLb = 0;
+};
+
+template <>
+void Solver<cpx<double>>::to_T(DMat<cpx<double>> dst, DMat<cpx<double>> src) { copy2d(dst, src, cx.st); }
+template <>
+void Solver<cpx<float>>::to_T(DMat<cpx<float>> dst, DMat<cpx<double>> src) { cast_c128_to_c64(dst, src, cx.st); }
+template <>
+void Solver<cpx<double>>::download(DMat<cpx<double>> src, double* host) {
+  HIPCHK(hipMemcpyAsync(host, src.p, sizeof(cpx<double>) * src.size(), hipMemcpyDeviceToHost, cx.st));
+  cx.sync();
+}
+template <>
+void Solver<cpx<float>>::download(DMat<cpx<float>> src, double* host) {
+  DMat<cpx<double>> t;
+  t.r = src.r; t.c = src.c; t.ld = src.r;
+  t.p = (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * src.size());
+  cast_c64_to_c128(t, src, cx.st);
+  HIPCHK(hipMemcpyAsync(host, t.p, sizeof(cpx<double>) * src.size(), hipMemcpyDeviceToHost, cx.st));
+  cx.sync();
+  cx.pool.free(t.p);
+}
+
+}  // namespace
+
+std::unique_ptr<SolverBase> make_solver(std::shared_ptr<DeviceGeometry> g, const SolverParams& p, int precision,
+                                        Ctx& cx) {
+  if (precision == PREC_C128) return std::unique_ptr<SolverBase>(new Solver<cpx<double>>(g, p, cx));
+  if (precision == PREC_C64) return std::unique_ptr<SolverBase>(new Solver<cpx<float>>(g, p, cx));
+  throw std::runtime_error("unknown precision");
+}
+
+}  // namespace rcs

@@ -1,0 +1,222 @@
+"""Pins of the FP64 CPU oracle (oracle/) against what the mathematics fixes.
+
+The reference ships no golden numbers (its only fixture is the 78-patch mesh), and it
+cannot be compiled here (Eigen3 is absent), so each oracle function is checked against
+closed forms, brute-force quadrature, physics (sphere capacitance, Mie series) or
+dense LU, chosen so that a dropped term, a wrong sign or a transposed operand fails.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import pyoracle as O
+from rcs_amd import mesh as M
+from tests.mie import pec_backscatter
+
+EPS0 = 8.854187817e-12
+C0 = 3.0e8
+
+
+def test_rwg_numbering_fixture78(fixture78):
+    nodes, tris = fixture78
+    m = O.Mesh(nodes, tris)
+    assert m.n_patch == 78 and m.n_rwg == 117  # every edge shared by two patches
+    t, ln = m.rwg()
+    keys = [tuple(r[:2]) for r in t]
+    assert keys == sorted(keys) and all(a < b for a, b in keys)  # std::map order (E:129-209)
+    for (n1, n2, tp, tm, fp, fm) in t:
+        for tri, fv in ((tp, fp), (tm, fm)):
+            s = set(tris[tri])
+            assert {n1, n2} <= s and fv in s and fv not in (n1, n2)
+        assert tp < tm  # T+ is the first patch in file order
+    full = np.vstack([np.zeros((1, 3)), nodes])
+    assert np.allclose(ln, np.linalg.norm(full[t[:, 0]] - full[t[:, 1]], axis=1))
+
+
+def _quad_triangle(f, a, b, c, depth=6):
+    """Brute force: split into 4**depth sub-triangles, 7-point Dunavant-like rule via
+    centroid sampling of a further split (converges like h^2)."""
+    tris = [(a, b, c)]
+    for _ in range(depth):
+        nt = []
+        for (p, q, r) in tris:
+            pq, qr, rp = (p + q) / 2, (q + r) / 2, (r + p) / 2
+            nt += [(p, pq, rp), (q, qr, pq), (r, rp, qr), (pq, qr, rp)]
+        tris = nt
+    P = np.array([t[0] for t in tris]); Q = np.array([t[1] for t in tris]); R = np.array([t[2] for t in tris])
+    area = 0.5 * np.linalg.norm(np.cross(Q - P, R - P), axis=1)
+    # 3-point edge-midpoint rule (degree 2) on every sub-triangle
+    pts = [(P + Q) / 2, (Q + R) / 2, (R + P) / 2]
+    return sum(area[:, None] * f(x) / 3.0 for x in pts).sum(axis=0)
+
+
+@pytest.mark.parametrize("h", [0.3, 1.0, 2.5])
+def test_panel_integrals_vs_quadrature(rng, h):
+    for _ in range(3):
+        a, b, c = rng.normal(size=(3, 3))
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        size = np.linalg.norm(b - a)
+        tc = (a + b + c) / 3 + rng.normal(size=3) * 0.3 * size + n * h * size
+        lap, rho = O.panel(tc, a, b, c)
+        cp = tc - np.dot(tc - a, n) * n  # projection of the observation point on the plane
+        f1 = lambda x: (1.0 / np.linalg.norm(x - tc, axis=1))[:, None]
+        fr = lambda x: (x - cp) / np.linalg.norm(x - tc, axis=1)[:, None]
+        ref1 = _quad_triangle(f1, a, b, c)[0]
+        refr = _quad_triangle(fr, a, b, c)
+        assert abs(lap - ref1) < 2e-5 * abs(ref1)
+        assert np.linalg.norm(rho - refr) < 2e-5 * np.linalg.norm(refr)
+
+
+def test_panel_integral_in_plane_point():
+    # observation point in the triangle's plane, outside it: 1/R integral stays finite
+    a, b, c = np.array([0., 0, 0]), np.array([1., 0, 0]), np.array([0., 1, 0])
+    tc = np.array([1.5, 1.2, 0.0])
+    lap, _ = O.panel(tc, a, b, c)
+    ref = _quad_triangle(lambda x: (1.0 / np.linalg.norm(x - tc, axis=1))[:, None], a, b, c)[0]
+    assert abs(lap - ref) < 1e-5 * ref
+
+
+def test_laplace_capacitance_sphere():
+    """Sphere of radius a: C = 4 pi eps0 a = 16.69 pF (a = 0.15 m), faceting error ~1%."""
+    nodes, tris = M.icosphere(3, 0.15)
+    m = O.Mesh(nodes, tris)
+    N = m.n_patch
+    sigma = m.dense_solve(O.LAPLACE, 1e9, np.ones(N, dtype=complex)).real
+    cap = float(np.dot(sigma, m.patches()[:, 3]))  # getCapacitance, src/ElectrostaticsMatrix.cpp:366-372
+    exact = 4 * np.pi * EPS0 * 0.15
+    assert abs(cap - exact) / exact < 0.02
+
+
+def test_mie_helper_rayleigh_limit():
+    a, x = 0.15, 0.02
+    assert abs(pec_backscatter(x, a) / (9 * np.pi * a * a * x ** 4) - 1) < 1e-3
+
+
+@pytest.mark.parametrize("freq,kind,tol_db", [
+    (3.0e8, O.EFIE, 0.2),                       # ka = 0.94: the reference's operator
+    (1.0e9, O.EFIE_PHYSICAL_PROJECTION, 0.5),  # ka = 3.14 with the origin-plane quirk isolated
+    (6.0e8, O.EFIE_PHYSICAL_PROJECTION, 0.5),
+])
+def test_efie_monostatic_rcs_vs_mie(freq, kind, tol_db):
+    """Dense EFIE solve with the physical plane wave (kvec = kappa z) vs the Mie series.
+    The reference projects the observation point onto the source plane shifted through
+    the origin (E:469-474); that error grows with ka (8.6 dB at ka = 3.14 on this mesh),
+    so the high-frequency check runs with the projection on the source plane."""
+    a = 0.15
+    kap = 2 * np.pi * freq / C0
+    nodes, tris = M.icosphere(3, a)
+    m = O.Mesh(nodes, tris)
+    b = m.rhs(freq, E=(1, 0, 0), kvec=(0, 0, kap))
+    x = m.dense_solve(kind, freq, b)
+    s = m.rcs(freq, x, np.array([0.0]), np.array([0.0]))[0]
+    mie = pec_backscatter(kap * a, a)
+    assert abs(10 * np.log10(s / mie)) < tol_db
+
+
+def test_aca_cur_error():
+    """ACA skeleton of a well-separated block reproduces it to about the tolerance."""
+    freq = 1.0e9
+    nodes, tris = M.icosphere(3, 0.15)
+    m = O.Mesh(nodes, tris)
+    loc = m.rwg_midpoints()
+    rows = np.where(loc[:, 2] > 0.135)[0]
+    cols = np.where(loc[:, 2] < -0.12)[0]
+    K = m.entries(O.EFIE, freq, rows, cols)
+    sv = np.linalg.svd(K, compute_uv=False)
+    for tol in (1e-3, 1e-5):
+        r, rb, cb = m.aca(O.EFIE, freq, rows, cols, tol)
+        assert 0 < r <= min(len(rows), len(cols))
+        assert (sv > 10 * tol * sv[0]).sum() <= r  # no premature stop
+        Ac, Ar, D = K[:, cb], K[rb, :], K[np.ix_(rb, cb)]
+        err = np.linalg.norm(K - Ac @ np.linalg.solve(D, Ar)) / np.linalg.norm(K)
+        assert err < 30 * tol
+        assert len(set(rb)) == r and len(set(cb)) == r
+
+
+def test_aca_exact_rank():
+    """Rows of a block that is exactly rank one are found after one or two pivots."""
+    freq = 1.0e9
+    nodes, tris = M.icosphere(1, 0.15)
+    m = O.Mesh(nodes, tris)
+    rows = np.arange(5)
+    cols = np.array([7, 7, 7, 7, 7, 7, 7, 7])  # identical columns: rank 1
+    r, rb, cb = m.aca(O.EFIE, freq, rows, cols, 1e-8)
+    assert r <= 2
+
+
+def test_rrqr_plain_cpqr_matches_lapack(rng):
+    A = rng.normal(size=(40, 25)) + 1j * rng.normal(size=(40, 25))
+    Q, R, perm, r, mp = O.rrqr(A, 14, 14, 0, 0.0, 1000)
+    _, _, piv = sla.qr(A, pivoting=True)
+    assert r == 25
+    assert list(perm[:r]) == list(piv[:r])  # same greedy max-norm pivot order
+    assert np.allclose(Q.conj().T @ Q, np.eye(r), atol=1e-12)
+    assert np.allclose(Q @ R, A[:, perm], atol=1e-11)
+    assert abs(mp - np.linalg.norm(A, axis=0).max()) < 1e-12 * mp
+
+
+def test_rrqr_partition_and_rank_cap(rng):
+    m, p, f = 60, 10, 30
+    B = np.linalg.qr(rng.normal(size=(m, p)) + 1j * rng.normal(size=(m, p)))[0]
+    F = rng.normal(size=(m, f)) + 1j * rng.normal(size=(m, f))
+    A = np.hstack([B, F])
+    for cap in (0, 3, 7):
+        Q, R, perm, r, mp = O.rrqr(A, 10, 10, p, 0.0, cap)
+        assert r == p + cap
+        assert set(perm[:p]) == set(range(p))  # old basis first
+        # the kept span contains the old basis exactly
+        assert np.linalg.norm(B - Q @ (Q.conj().T @ B)) < 1e-12
+    # tolerance stop in the second partition: fill-in of numerical rank 4
+    F2 = (rng.normal(size=(m, 4)) @ rng.normal(size=(4, f))).astype(complex)
+    Q, R, perm, r, mp = O.rrqr(np.hstack([B, F2]), 10, 10, p, 0.0, 1000)
+    assert r == p + 4
+
+
+def test_rrqr_first_partition_early_stop_keeps_rank_minus_one():
+    """src/rrqr.cpp:115-117: a pivot ratio below tol1 at step j sets the rank to j-1."""
+    m = 30
+    q = np.linalg.qr(np.random.default_rng(5).normal(size=(m, 3)))[0].astype(complex)
+    # columns 0 and 2 have equal norms to double precision, so column 0 pivots first
+    B = np.column_stack([q[:, 0] * 3, q[:, 1] * 2, q[:, 0] * 3 + 1e-9 * q[:, 2]])
+    F = np.zeros((m, 2), dtype=complex)
+    # step 2 stops (ratio ~3e-10) with rank1 = 1, not 2; with no fill-in budget the
+    # second partition adds nothing, so the kept rank is 1
+    Q, R, perm, r, mp = O.rrqr(np.hstack([B, F]), 6, 6, 3, 0.0, 0)
+    assert r == 1
+    # with budget, the second partition restarts at position 1 and re-accepts the
+    # dropped direction (pivot ratio 1), then stops on the ~1e-9 column: rank 2
+    Q, R, perm, r, mp = O.rrqr(np.hstack([B, F]), 6, 6, 3, 0.0, 5)
+    assert r == 2
+    assert np.linalg.norm(q[:, :2] - Q @ (Q.conj().T @ q[:, :2])) < 1e-12
+
+
+def _aifmm_vs_dense(kind, freq, tols):
+    """Elongated ellipsoid: the cubic octree puts ~80-130 unknowns in each non-empty leaf,
+    so skeletons are genuinely low rank and the RRQR fill-in path runs."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    m = O.Mesh(nodes, tris)
+    loc = m.centroids() if kind == O.LAPLACE else m.rwg_midpoints()
+    b = np.ones(len(loc), dtype=complex) if kind == O.LAPLACE else m.rhs(freq)
+    xd = m.dense_solve(kind, freq, b)
+    out = []
+    for tp in tols:
+        s = O.AIFMM(m, kind, freq, loc, M.bounding_box(loc), 64, tp, tp)
+        s.factorize()
+        x = s.solve(b)
+        out.append((np.linalg.norm(x - xd) / np.linalg.norm(xd), s.info(), s.ranks(s.info()["nLevels"])))
+    return out
+
+
+def test_aifmm_laplace_converges_to_dense():
+    (e5, i5, r5), (e10, i10, r10) = _aifmm_vs_dense(O.LAPLACE, 1e9, (5, 10))
+    assert i5["n_rrqr"] > 0 and i10["n_rrqr"] > 0
+    nz = r5[:, 2] > 0
+    assert (r5[nz, 0] < r5[nz, 2]).all()  # every non-empty leaf is compressed
+    assert e5 < 1e-3 and e10 < 3e-5 and e10 < e5 / 5
+
+
+def test_aifmm_efie_with_compressions():
+    (e5, i5, r5), = _aifmm_vs_dense(O.EFIE, 1e9, (5,))
+    assert i5["n_rrqr"] > 0 and i5["n_compress_p2p"] + i5["n_compress_p2l"] > 0
+    assert e5 < 1e-2
