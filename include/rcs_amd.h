@@ -97,6 +97,22 @@ int rcs_solver_stats(rcs_solver_t s, int* info, double* vals);
  * the number of boxes written, or a negative status. */
 int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap);
 
+/* Read one stored block of the tree (FMM2DBox members, include/AIFMMBox.hpp:45-59):
+ * which = 0 L2P, 1 P2M, 2 M2L[key], 3 P2P[key], 4 M2P[key], 5 P2L[key], 6 L2M_f.
+ * shape receives rows, cols; out holds cap complex values. Returns 1 when the block
+ * does not exist. */
+int rcs_solver_block(rcs_solver_t s, int level, int box, int which, int key, int* shape, double* out,
+                     long long cap);
+
+/* Phase timing with HIP events on the library stream. No counterpart in the reference
+ * (its driver prints omp_get_wtime phases, T:4610-4744); used by bench.py for the
+ * roofline figure of the dominant kernel. Tags: 0 Schur fill-in GEMMs, 1 GETRF,
+ * 2 GETRS, 3 entry generation, 4 ACA, 5 RRQR, 6 basis updates. work = flops
+ * (entries for tag 3). */
+int rcs_profile_enable(int on);
+int rcs_profile_reset(void);
+int rcs_profile_get(int tag, double* work, double* ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif

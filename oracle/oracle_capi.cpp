@@ -5,6 +5,7 @@
 // arrays are interleaved (re, im) doubles, matrices column-major.
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include "oracle_tree.hpp"
 
@@ -225,6 +226,8 @@ int orc_aifmm_factorize(void* h) {
   ORC_TRY
   OSolver* s = (OSolver*)h;
   s->S->A->ctr = Counters();
+  const char* e = std::getenv("RCS_DEBUG_STOP");
+  if (e) std::sscanf(e, "%d:%d", &s->S->A->stopLevel, &s->S->A->stopBox);
   double t0 = now();
   s->S->factorize();
   s->t_factor = now() - t0;
@@ -277,3 +280,38 @@ int orc_aifmm_ranks(void* h, int j, int* out) {
 }
 
 }  // extern "C"
+
+extern "C" int orc_aifmm_block(void* h, int level, int box, int which, int key, int* shape, double* out,
+                               long long cap) {
+  Tree* t = ((OSolver*)h)->S->A;
+  if (level < 0 || level > t->nLevels || box < 0 || box >= t->nBoxesPerLevel[level]) return 1;
+  Box& b = t->B(level, box);
+  const Mat* m = nullptr;
+  auto find = [&](std::map<int, Mat>& mp) -> const Mat* {
+    auto it = mp.find(key);
+    return it == mp.end() ? nullptr : &it->second;
+  };
+  switch (which) {
+    case 0: m = &b.L2P; break;
+    case 1: m = &b.P2M; break;
+    case 2: m = find(b.M2L); break;
+    case 3: m = find(b.P2P); break;
+    case 4: m = find(b.M2P); break;
+    case 5: m = find(b.P2L); break;
+    case 6: m = &b.L2M_f; break;
+    case 7: m = &b.particle_rhs; break;
+    case 8: m = &b.multipole_rhs; break;
+    case 9: m = &b.local_rhs; break;
+    case 10: m = &b.particles; break;
+    case 11: m = &b.multipoles; break;
+    case 12: m = &b.locals; break;
+    case 13: m = find(b.L2P_f); break;
+    default: return 1;
+  }
+  if (!m) return 1;
+  shape[0] = m->r;
+  shape[1] = m->c;
+  if ((long long)m->a.size() > cap) return -1;
+  std::memcpy(out, m->a.data(), sizeof(cd) * m->a.size());
+  return 0;
+}

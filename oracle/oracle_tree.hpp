@@ -12,6 +12,8 @@
 // T:4825, T:4843), and solve(P2P[k][pp]) is formed once per pp instead of once per
 // (nn, pp) (T:3934-4019 recompute the identical product).
 #pragma once
+#include <cstdio>
+#include <cstdlib>
 #include "oracle_lowrank.hpp"
 
 namespace oracle {
@@ -629,13 +631,19 @@ struct Tree {
   void compress_P2P_myRRQR(int j, int k, int b, const Mat& oldL2P, const Mat& oldP2M, const Mat& F,
                            Mat& newL2P, Mat& newP2Mt, Mat& M2Lkb) {
     Mat Acol = hcat(oldL2P, F);
+    if (std::getenv("RCS_DEBUG_LOG"))
+      std::fprintf(stderr, "P2P j%d k%d b%d A %dx%d part %d ip %.6e nF %.6e\n", j, k, b, Acol.r, Acol.c, oldL2P.c,
+                   B(j, k).maxPivot_L2P, F.norm());
     RRQRResult r1 = rrqr(Acol, TOL_POW, fill_in_TOL_POW, oldL2P.c, B(j, k).maxPivot_L2P, fillin_rankVec[j], &ctr);
+    if (std::getenv("RCS_DEBUG_LOG")) std::fprintf(stderr, "  r1 %d mp %.6e\n", r1.rank, r1.maxPivot);
     B(j, k).maxPivot_L2P = std::max(B(j, k).maxPivot_L2P, r1.maxPivot);
     newL2P = r1.Q;
     Mat Rk = gemm(adjoint(newL2P), oldL2P);
     Mat Rkp = gemm(adjoint(newL2P), F);
     Mat Arow = hcat(adjoint(oldP2M), adjoint(Rkp));
     RRQRResult r2 = rrqr(Arow, TOL_POW, fill_in_TOL_POW, oldP2M.r, B(j, b).maxPivot_P2M, fillin_rankVec[j], &ctr);
+    if (std::getenv("RCS_DEBUG_LOG"))
+      std::fprintf(stderr, "  r2 %d mp %.6e ip %.6e A %dx%d\n", r2.rank, r2.maxPivot, B(j, b).maxPivot_P2M, Arow.r, Arow.c);
     B(j, b).maxPivot_P2M = std::max(B(j, b).maxPivot_P2M, r2.maxPivot);
     newP2Mt = r2.Q;
     Mat Rn = gemm(adjoint(newP2Mt), adjoint(oldP2M));
@@ -755,9 +763,11 @@ struct Tree {
         B(j, k).Eliminated = true;
         eliminate_x(j, k);
         eliminate_z(j, k);
+        if (j == stopLevel && k == stopBox) return;
       }
     }
   }
+  int stopLevel = -1, stopBox = -1;  // debugging: stop right after eliminating (stopLevel, stopBox)
 
   // ---------------- solve (T:670-678, 3506-3535, 3613-3653, 4115-4148, 4749-4907) ----------------
   void assign_Leaf_rhs(const std::vector<cd>& b) {

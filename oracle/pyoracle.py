@@ -242,3 +242,18 @@ def panel(tc, a, b, c):
     rho = np.zeros(3)
     L.orc_panel(*[_dp(x) for x in arr], _dp(lap), _dp(rho))
     return float(lap[0]), rho
+
+
+def aifmm_block(solver, level, box, which, key=0, cap=4_000_000):
+    """One stored block of the oracle tree (see rcs_solver_block for `which`)."""
+    L = lib()
+    L.orc_aifmm_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double), C.c_longlong]
+    shape = np.zeros(2, dtype=np.int32)
+    out = np.zeros(cap, dtype=np.complex128)
+    rc = L.orc_aifmm_block(solver.h, level, box, which, key, _ip(shape), out.ctypes.data_as(C.POINTER(C.c_double)),
+                           cap)
+    if rc != 0:
+        return None
+    r, c = int(shape[0]), int(shape[1])
+    return out[: r * c].reshape((r, c), order="F").copy()

@@ -53,6 +53,8 @@ def lib():
         L.rcs_solver_solve.argtypes = [vp, dp, C.c_int, dp, dp]
         L.rcs_solver_stats.argtypes = [vp, ip, dp]
         L.rcs_solver_ranks.argtypes = [vp, C.c_int, ip, C.c_int]
+        L.rcs_profile_enable.argtypes = [C.c_int]
+        L.rcs_profile_get.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_longlong)]
         _LIB = L
     return _LIB
 
@@ -61,8 +63,29 @@ EXPORTED = [
     "rcs_init", "rcs_last_error", "rcs_mesh_create", "rcs_mesh_destroy", "rcs_mesh_size", "rcs_mesh_locations",
     "rcs_mesh_rwg", "rcs_entries", "rcs_rhs", "rcs_rcs", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
     "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
-    "rcs_solver_ranks",
+    "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
 ]
+
+PROFILE_TAGS = ["schur", "getrf", "getrs", "entries", "aca", "rrqr", "update"]
+
+
+def set_profiling(on=True):
+    _check(lib().rcs_profile_enable(1 if on else 0))
+
+
+def reset_profile():
+    _check(lib().rcs_profile_reset())
+
+
+def profile():
+    """Event-timed totals per phase: {tag: {"flops"|"work", "ms", "launches"}}."""
+    out = {}
+    for i, t in enumerate(PROFILE_TAGS):
+        w, ms = C.c_double(), C.c_double()
+        n = C.c_longlong()
+        _check(lib().rcs_profile_get(i, C.byref(w), C.byref(ms), C.byref(n)))
+        out[t] = {"flops": w.value, "ms": ms.value, "launches": n.value}
+    return out
 
 
 def _dp(a):
@@ -244,3 +267,18 @@ class AIFMM:
         if n < 0:
             _check(n)
         return out[: 3 * n].reshape(n, 3)
+
+
+def solver_block(solver, level, box, which, key=0, cap=4_000_000):
+    """One stored block of the GPU tree (which: 0 L2P, 1 P2M, 2 M2L, 3 P2P, 4 M2P, 5 P2L, 6 L2M_f)."""
+    L = lib()
+    L.rcs_solver_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.c_longlong]
+    shape = np.zeros(2, dtype=np.int32)
+    out = np.zeros(cap, dtype=np.complex128)
+    rc = L.rcs_solver_block(solver.h, level, box, which, key, _ip(shape), _cplx_ptr(out), cap)
+    if rc == 1:
+        return None
+    _check(rc)
+    r, c = int(shape[0]), int(shape[1])
+    return out[: r * c].reshape((r, c), order="F").copy()

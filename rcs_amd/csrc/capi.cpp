@@ -378,6 +378,38 @@ int rcs_solver_stats(rcs_solver_t s, int* info, double* vals) {
   return 0;
   CATCH
 }
+int rcs_solver_block(rcs_solver_t s, int level, int box, int which, int key, int* shape, double* out, long long cap) {
+  TRY int r = 0, c = 0;
+  std::vector<double> v;
+  if (!s->s->block(level, box, which, key, r, c, v)) return 1;
+  shape[0] = r;
+  shape[1] = c;
+  if ((long long)v.size() > 2 * cap) throw std::runtime_error("block buffer too small");
+  std::memcpy(out, v.data(), sizeof(double) * v.size());
+  return 0;
+  CATCH
+}
+int rcs_profile_enable(int on) {
+  TRY ctx().prof.on = (on != 0);
+  return 0;
+  CATCH
+}
+int rcs_profile_reset(void) {
+  TRY Ctx& cx = ctx();
+  cx.prof.reset(cx.st);
+  return 0;
+  CATCH
+}
+int rcs_profile_get(int tag, double* work, double* ms, long long* launches) {
+  TRY if (tag < 0 || tag >= PROF_NTAGS) throw std::runtime_error("unknown profile tag");
+  Ctx& cx = ctx();
+  Profiler::Agg a = cx.prof.get(cx.st, tag);
+  *work = a.work;
+  *ms = a.ms;
+  *launches = a.launches;
+  return 0;
+  CATCH
+}
 int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap) {
   TRY std::vector<int> v;
   s->s->ranks(level, v);

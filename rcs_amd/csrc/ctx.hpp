@@ -1,6 +1,7 @@
 // Execution context: one HIP stream, a caching HBM block pool and a staging ring for
 // launch parameter arrays (pinned host memory mirrored into HBM in stream order).
 #pragma once
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -18,7 +19,7 @@ class DevPool {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~size_t(255);
     auto it = free_.lower_bound(bytes);
-    if (it != free_.end() && it->first <= bytes + bytes / 4) {
+    if (!no_reuse_ && it != free_.end() && it->first <= bytes + bytes / 4) {
       void* p = it->second;
       size_t sz = it->first;
       free_.erase(it);
@@ -64,11 +65,82 @@ class DevPool {
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> used_;
   size_t total_ = 0, in_use_ = 0, peak_ = 0;
+  bool no_reuse_ = std::getenv("RCS_DEBUG_NO_REUSE") != nullptr;  // debugging: never recycle blocks
+};
+
+// Phase timing with HIP events on the library's stream (off unless enabled). Each
+// record brackets the launches of one tagged operation; totals are folded in at
+// read-out (or when the event pool fills up).
+enum ProfTag { PROF_SCHUR = 0, PROF_GETRF = 1, PROF_GETRS = 2, PROF_ENTRIES = 3, PROF_ACA = 4, PROF_RRQR = 5,
+               PROF_UPDATE = 6, PROF_NTAGS = 7 };
+class Profiler {
+ public:
+  struct Agg {
+    double work = 0, ms = 0;
+    long long launches = 0;
+  };
+  bool on = false;
+  ~Profiler() {
+    for (auto& e : events_) (void)hipEventDestroy(e);
+  }
+  int begin(hipStream_t st) {
+    if (!on) return -1;
+    int i = take();
+    HIPCHK(hipEventRecord(events_[i], st));
+    return i;
+  }
+  void end(hipStream_t st, int h, int tag, double work) {
+    if (!on || h < 0) return;
+    int i = take();
+    HIPCHK(hipEventRecord(events_[i], st));
+    recs_.push_back({h, i, tag, work});
+    if (used_ > 4000) fold(st);
+  }
+  void fold(hipStream_t st) {
+    if (recs_.empty()) { used_ = 0; return; }
+    HIPCHK(hipStreamSynchronize(st));
+    for (auto& r : recs_) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, events_[r.a], events_[r.b]));
+      agg_[r.tag].ms += ms;
+      agg_[r.tag].work += r.work;
+      agg_[r.tag].launches += 1;
+    }
+    recs_.clear();
+    used_ = 0;
+  }
+  void reset(hipStream_t st) {
+    fold(st);
+    for (auto& a : agg_) a = Agg();
+  }
+  Agg get(hipStream_t st, int tag) {
+    fold(st);
+    return agg_[tag];
+  }
+
+ private:
+  struct Rec {
+    int a, b, tag;
+    double work;
+  };
+  int take() {
+    if (used_ == (int)events_.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      events_.push_back(e);
+    }
+    return used_++;
+  }
+  std::vector<hipEvent_t> events_;
+  std::vector<Rec> recs_;
+  int used_ = 0;
+  Agg agg_[PROF_NTAGS];
 };
 
 struct Ctx {
   hipStream_t st = nullptr;
   DevPool pool;
+  Profiler prof;
   char* h_stage = nullptr;
   char* d_stage = nullptr;
   size_t cap = 0, off = 0;

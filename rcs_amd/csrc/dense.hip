@@ -1,4 +1,6 @@
 // Dense complex kernels for gfx950 (CDNA4, 64-wide wavefronts).
+#include <algorithm>
+
 #include "dense.hpp"
 
 namespace rcs {
@@ -21,24 +23,40 @@ __device__ inline T ld_op(const T* A, int lda, int i, int k, int op) {
 
 // ---------------------------------------------------------------- grouped GEMM
 // 64x64 output tile per 256-thread workgroup, K staged through LDS in chunks of 16.
-// Thread (tx, ty) accumulates rows ty+16i and columns tx+16j (i, j < 4).
+// Thread (tx, ty) accumulates rows ty+16i and columns tx+16j (i, j < 4). An output
+// block may sum several products (segments); the tile walks them all before writing.
 constexpr int GT = 64, GK = 16;
 
 template <typename T>
-__global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOp<T>* ops, int nops) {
+struct GemmSeg {  // one product op(A)*op(B) of an output block
+  const T* A;
+  const T* B;
+  int lda, ldb, k, opA, opB;
+};
+template <typename T>
+struct GemmOut {  // one output block
+  T* C;
+  int ldc, m, n;
+  T alpha, beta;
+  int tile0, seg0, nseg;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOut<T>* outs, int nouts,
+                                                           const GemmSeg<T>* segs) {
   using R = typename RealOf<T>::type;
   __shared__ T As[GK][GT + 1];
   __shared__ T Bs[GK][GT + 1];
   const int tile = blockIdx.x;
-  int lo = 0, hi = nops - 1;
+  int lo = 0, hi = nouts - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].tile0 <= tile) lo = mid;
+    if (outs[mid].tile0 <= tile) lo = mid;
     else hi = mid - 1;
   }
-  const GemmOp<T> op = ops[lo];
-  const int t = tile - op.tile0;
-  const int tm = (op.m + GT - 1) / GT;
+  const GemmOut<T> out = outs[lo];
+  const int t = tile - out.tile0;
+  const int tm = (out.m + GT - 1) / GT;
   const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   T acc[4][4];
@@ -46,49 +64,52 @@ __global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOp<T>* ops,
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
-  for (int k0 = 0; k0 < op.k; k0 += GK) {
+  for (int sgi = 0; sgi < out.nseg; ++sgi) {
+    const GemmSeg<T> op = segs[out.seg0 + sgi];
+    for (int k0 = 0; k0 < op.k; k0 += GK) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      int i, kk;
-      if (op.opA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * s; }
-      else { kk = tid & 15; i = (tid >> 4) + 16 * s; }
-      T v(0, 0);
-      if (m0 + i < op.m && k0 + kk < op.k) v = ld_op(op.A, op.lda, m0 + i, k0 + kk, op.opA);
-      As[kk][i] = v;
-      int j2, k2;
-      if (op.opB == OP_N) { k2 = tid & 15; j2 = (tid >> 4) + 16 * s; }
-      else { j2 = tid & 63; k2 = (tid >> 6) + 4 * s; }
-      T w(0, 0);
-      if (n0 + j2 < op.n && k0 + k2 < op.k) w = ld_op(op.B, op.ldb, k0 + k2, n0 + j2, op.opB);
-      Bs[k2][j2] = w;
-    }
-    __syncthreads();
+      for (int s = 0; s < 4; ++s) {
+        int i, kk;
+        if (op.opA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * s; }
+        else { kk = tid & 15; i = (tid >> 4) + 16 * s; }
+        T v(0, 0);
+        if (m0 + i < out.m && k0 + kk < op.k) v = ld_op(op.A, op.lda, m0 + i, k0 + kk, op.opA);
+        As[kk][i] = v;
+        int j2, k2;
+        if (op.opB == OP_N) { k2 = tid & 15; j2 = (tid >> 4) + 16 * s; }
+        else { j2 = tid & 63; k2 = (tid >> 6) + 4 * s; }
+        T w(0, 0);
+        if (n0 + j2 < out.n && k0 + k2 < op.k) w = ld_op(op.B, op.ldb, k0 + k2, n0 + j2, op.opB);
+        Bs[k2][j2] = w;
+      }
+      __syncthreads();
 #pragma unroll 4
-    for (int kk = 0; kk < GK; ++kk) {
-      T a[4], b[4];
+      for (int kk = 0; kk < GK; ++kk) {
+        T a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+        for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cfma(acc[i][j], a[i], b[j]);
+          for (int j = 0; j < 4; ++j) cfma(acc[i][j], a[i], b[j]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  const bool zb = (op.beta.re == R(0) && op.beta.im == R(0));
+  const bool zb = (out.beta.re == R(0) && out.beta.im == R(0));
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     int r = m0 + ty + 16 * i;
-    if (r >= op.m) continue;
+    if (r >= out.m) continue;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       int c = n0 + tx + 16 * j;
-      if (c >= op.n) continue;
-      T* cp = op.C + (size_t)r + (size_t)c * op.ldc;
-      T v = op.alpha * acc[i][j];
-      if (!zb) v = v + op.beta * (*cp);
+      if (c >= out.n) continue;
+      T* cp = out.C + (size_t)r + (size_t)c * out.ldc;
+      T v = out.alpha * acc[i][j];
+      if (!zb) v = v + out.beta * (*cp);
       *cp = v;
     }
   }
@@ -96,23 +117,44 @@ __global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOp<T>* ops,
 
 template <typename T>
 void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx) {
-  hipStream_t st = cx.st;
   if (ops.empty()) return;
+  std::vector<GemmOut<T>> outs;
+  std::vector<GemmSeg<T>> segs;
+  outs.reserve(ops.size());
+  segs.reserve(ops.size());
   int tiles = 0;
-  std::vector<GemmOp<T>> live;
-  live.reserve(ops.size());
-  for (auto& o : ops) {
+  for (const auto& o : ops) {
     if (o.m <= 0 || o.n <= 0) continue;
-    if (o.k <= 0) {  // pure scaling by beta: handled as a K=0 tile pass
-      o.k = 0;
+    const bool join = o.chain && !outs.empty() && outs.back().C == o.C;
+    if (join) {
+      const GemmOut<T>& h = outs.back();
+      if (h.m != o.m || h.n != o.n || h.ldc != o.ldc || h.alpha.re != o.alpha.re || h.alpha.im != o.alpha.im)
+        throw std::runtime_error("gemm_grouped: chained op differs from its output block");
+    } else {
+      GemmOut<T> h{};
+      h.C = o.C; h.ldc = o.ldc; h.m = o.m; h.n = o.n;
+      h.alpha = o.alpha; h.beta = o.beta;
+      h.tile0 = tiles; h.seg0 = (int)segs.size(); h.nseg = 0;
+      tiles += ((o.m + GT - 1) / GT) * ((o.n + GT - 1) / GT);
+      outs.push_back(h);
     }
-    o.tile0 = tiles;
-    tiles += ((o.m + GT - 1) / GT) * ((o.n + GT - 1) / GT);
-    live.push_back(o);
+    if (o.k > 0) {  // k = 0 contributes nothing: the block becomes beta*C
+      segs.push_back(GemmSeg<T>{o.A, o.B, o.lda, o.ldb, o.k, o.opA, o.opB});
+      outs.back().nseg++;
+    }
   }
-  if (live.empty()) return;
-  const GemmOp<T>* d_ops = (const GemmOp<T>*)cx.push(live.data(), sizeof(GemmOp<T>) * live.size());
-  hipLaunchKernelGGL(gemm_grouped_kernel<T>, dim3(tiles), dim3(256), 0, st, d_ops, (int)live.size());
+  if (outs.empty()) return;
+  if (outs.size() > 1) {  // workgroups of distinct output blocks run concurrently
+    std::vector<const T*> cs(outs.size());
+    for (size_t i = 0; i < outs.size(); ++i) cs[i] = outs[i].C;
+    std::sort(cs.begin(), cs.end());
+    if (std::adjacent_find(cs.begin(), cs.end()) != cs.end())
+      throw std::runtime_error("gemm_grouped: two output blocks of one launch share C (chain them)");
+  }
+  if (segs.empty()) segs.push_back(GemmSeg<T>{});
+  const GemmOut<T>* d_outs = (const GemmOut<T>*)cx.push(outs.data(), sizeof(GemmOut<T>) * outs.size());
+  const GemmSeg<T>* d_segs = (const GemmSeg<T>*)cx.push(segs.data(), sizeof(GemmSeg<T>) * segs.size());
+  hipLaunchKernelGGL(gemm_grouped_kernel<T>, dim3(tiles), dim3(256), 0, cx.st, d_outs, (int)outs.size(), d_segs);
   HIPCHK(hipGetLastError());
 }
 

@@ -30,10 +30,15 @@ struct GemmOp {
   int m, n, k;
   int opA, opB;
   T alpha, beta;
-  int tile0;
+  // chain = 1: accumulate into the same C as the previous op of the group (same m, n,
+  // alpha); the products are summed inside one tile and C is written once, so no two
+  // workgroups ever update the same output block. An op whose C differs from the
+  // previous op's starts a new output block with its own beta.
+  int chain;
 };
 
-// C_g = alpha*op(A_g)*op(B_g) + beta*C_g for every op of the group, in one launch.
+// C_g = alpha*sum_s op(A_s)*op(B_s) + beta*C_g for every output block of the group, in
+// one launch. Two output blocks of one group must not share C (checked on the host).
 template <typename T>
 void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx);
 

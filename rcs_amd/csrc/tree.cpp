@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace rcs {
@@ -109,8 +111,9 @@ class Solver : public SolverBase {
   struct Group {
     std::vector<GemmOp<T>> ops;
   };
-  // C = alpha*op(A)*op(B) + beta*C
-  void add(Group& g, DMat<T> C, DMat<T> A, int opA, DMat<T> B, int opB, T alpha, T beta) {
+  // C = alpha*op(A)*op(B) + beta*C. chain = 1 sums into the same C as the previous op
+  // of the group inside one output tile (several products into one block).
+  void add(Group& g, DMat<T> C, DMat<T> A, int opA, DMat<T> B, int opB, T alpha, T beta, int chain = 0) {
     int m = opA == OP_N ? A.r : A.c, k = opA == OP_N ? A.c : A.r;
     int kb = opB == OP_N ? B.r : B.c, n = opB == OP_N ? B.c : B.r;
     if (k != kb || C.r != m || C.c != n) throw std::runtime_error("gemm: shape mismatch");
@@ -126,10 +129,15 @@ class Solver : public SolverBase {
     o.C = C.p; o.ldc = C.ld;
     o.m = m; o.n = n; o.k = k;
     o.alpha = alpha; o.beta = beta;
+    o.chain = chain;
     g.ops.push_back(o);
   }
-  void run(Group& g) {
+  void run(Group& g, int tag = -1) {
+    double fl = 0;
+    for (auto& o : g.ops) fl += 8.0 * o.m * (double)o.n * o.k;
+    int h = tag >= 0 ? cx.prof.begin(cx.st) : -1;
     gemm_grouped(g.ops, cx);
+    if (tag >= 0) cx.prof.end(cx.st, h, tag, fl);
     g.ops.clear();
   }
   DMat<T> mul(DMat<T> A, int opA, DMat<T> B, int opB, T alpha = T(1, 0)) {
@@ -155,13 +163,17 @@ class Solver : public SolverBase {
     if (ip) cx.pool.free(ip);
     ip = (int*)cx.pool.alloc(sizeof(int) * std::max(A.r, 1));
     double t0 = wall();
+    int h = cx.prof.begin(cx.st);
     getrf(lu, ip, cx);
+    cx.prof.end(cx.st, h, PROF_GETRF, 8.0 / 3.0 * A.r * (double)A.r * A.r);
     stats_.t_getrf += wall() - t0;
     stats_.getrf_flops += 8.0 / 3.0 * A.r * (double)A.r * A.r;
   }
   void solve_inplace(DMat<T> lu, const int* ip, DMat<T> B) {
     stats_.getrs_flops += 8.0 * lu.r * (double)lu.r * B.c;
+    int h = cx.prof.begin(cx.st);
     getrs(lu, ip, B, cx);
+    cx.prof.end(cx.st, h, PROF_GETRS, 8.0 * lu.r * (double)lu.r * B.c);
   }
   std::vector<double> norms(const std::vector<DMat<T>>& v) {
     std::vector<double> out(v.size(), 0.0);
@@ -200,8 +212,12 @@ class Solver : public SolverBase {
       if (tiles > (1 << 28)) throw std::runtime_error("entries: launch too large");
     }
     if (!tasks.empty()) {
+      double ne = 0;
+      for (auto& t : tasks) ne += (double)t.nr * t.nc;
       const EntryTask<T>* d = (const EntryTask<T>*)cx.push(tasks.data(), sizeof(EntryTask<T>) * tasks.size());
+      int h = cx.prof.begin(cx.st);
       launch_entries<T>(G->d, G->h.kind, d, (int)tasks.size(), tiles, cx.st);
+      cx.prof.end(cx.st, h, PROF_ENTRIES, ne);
     }
   }
 
@@ -332,7 +348,9 @@ class Solver : public SolverBase {
           if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
         }
         std::vector<AcaResult> res;
+        int h = cx.prof.begin(cx.st);
         aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
+        cx.prof.end(cx.st, h, PROF_ACA, 0.0);
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
         std::vector<std::vector<int>> pivcols(nb);
@@ -359,7 +377,9 @@ class Solver : public SolverBase {
           if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
         }
         std::vector<AcaResult> res;
+        int h = cx.prof.begin(cx.st);
         aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
+        cx.prof.end(cx.st, h, PROF_ACA, 0.0);
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
         std::vector<std::vector<int>> pivrows(nb);
@@ -659,7 +679,7 @@ class Solver : public SolverBase {
           assign_or_sub(g, C, Ank, OP_N, Xn[pp], OP_N);
         }
       }
-      run(g);
+      run(g, PROF_SCHUR);
       stats_.t_schur += wall() - t0;
       drop(X);
     } else {  // T:3656-3791: zero blocks of the right shapes
@@ -730,7 +750,7 @@ class Solver : public SolverBase {
           add(g, C, Lf, OP_N, Yn[pp], OP_N, T(-1, 0), T(1, 0));
         }
       }
-      run(g);
+      run(g, PROF_SCHUR);
       stats_.t_schur += wall() - t0;
       drop(Y);
       // fill-in pairs to recompress (T:4413-4418, 4445-4453), pushed in loop order
@@ -787,8 +807,10 @@ class Solver : public SolverBase {
     DMat<T> W = clone(A);
     DMat<T> Qb = newmat(A.r, std::max(1, std::min(A.r, A.c)));
     double t0 = wall();
+    int h = cx.prof.begin(cx.st);
     RRQRInfo inf = rrqr(W, partition, std::pow(10.0, -TOL_POW), std::pow(10.0, -fill_in_TOL_POW), inputPivot,
                         rankVec[j], Qb, cx);
+    cx.prof.end(cx.st, h, PROF_RRQR, 0.0);
     stats_.t_compress += wall() - t0;
     stats_.n_rrqr++;
     double r = inf.rank, m = A.r, n = A.c;
@@ -830,7 +852,7 @@ class Solver : public SolverBase {
       add(g, Y, R, OP_N, *X, OP_N, T(1, 0), T(0, 0));
       fresh.push_back(Y);
     }
-    run(g);
+    run(g, PROF_UPDATE);
     for (size_t i = 0; i < tg.size(); ++i) put(*tg[i], fresh[i]);
     drop(R);
   }
@@ -852,7 +874,7 @@ class Solver : public SolverBase {
       add(g, Y, *X, OP_N, R, OP_N, T(1, 0), T(0, 0));
       fresh.push_back(Y);
     }
-    run(g);
+    run(g, PROF_UPDATE);
     for (size_t i = 0; i < tg.size(); ++i) put(*tg[i], fresh[i]);
     drop(R);
   }
@@ -904,7 +926,13 @@ class Solver : public SolverBase {
   void compress_P2P_myRRQR(int j, int k, int b, DMat<T> oldL2P, DMat<T> oldP2M, DMat<T> F, DMat<T>& newL2P,
                            DMat<T>& newP2Mt, DMat<T>& M2Lkb) {
     DMat<T> Acol = hcat2(oldL2P, OP_N, F, OP_N);
+    if (std::getenv("RCS_DEBUG_LOG")) {
+      double nf = norms({F})[0];
+      std::fprintf(stderr, "P2P j%d k%d b%d A %dx%d part %d ip %.6e nF %.6e\n", j, k, b, Acol.r, Acol.c, oldL2P.c,
+                   tree[j][k].maxPivot_L2P, nf);
+    }
     QR q1 = do_rrqr(Acol, oldL2P.c, tree[j][k].maxPivot_L2P, j);
+    if (std::getenv("RCS_DEBUG_LOG")) std::fprintf(stderr, "  r1 %d mp %.6e\n", q1.Q.c, q1.maxPivot);
     drop(Acol);
     tree[j][k].maxPivot_L2P = std::max(tree[j][k].maxPivot_L2P, q1.maxPivot);
     newL2P = q1.Q;
@@ -912,6 +940,8 @@ class Solver : public SolverBase {
     DMat<T> Rkp = mul(newL2P, OP_C, F, OP_N);
     DMat<T> Arow = hcat2(oldP2M, OP_C, Rkp, OP_C);
     QR q2 = do_rrqr(Arow, oldP2M.r, tree[j][b].maxPivot_P2M, j);
+    if (std::getenv("RCS_DEBUG_LOG"))
+      std::fprintf(stderr, "  r2 %d mp %.6e ip %.6e A %dx%d\n", q2.Q.c, q2.maxPivot, tree[j][b].maxPivot_P2M, Arow.r, Arow.c);
     drop(Arow);
     tree[j][b].maxPivot_P2M = std::max(tree[j][b].maxPivot_P2M, q2.maxPivot);
     newP2Mt = q2.Q;
@@ -1029,6 +1059,7 @@ class Solver : public SolverBase {
 
   void factorize() override {  // T:4599-4747
     double t0 = wall();
+    read_debug_env();
     for (int j = nLevels; j >= startLevel; --j) {
       if (j != nLevels) {
         initialise_P2P_NonLeafLevel(j);
@@ -1064,6 +1095,10 @@ class Solver : public SolverBase {
         tree[j][k].Eliminated = true;
         eliminate_x(j, k);
         eliminate_z(j, k);
+        if (j == dbg_level && k == dbg_box) {  // debugging aid: stop after this box
+          cx.sync();
+          return;
+        }
       }
     }
     factor_base_level();
@@ -1210,8 +1245,8 @@ class Solver : public SolverBase {
           for (int nn : bk.NB) {
             if (nn == k) continue;
             Box<T>& bn = tree[j][nn];
-            if (!bn.Eliminated) add(g, rhs1, bk.P2P[nn], OP_N, bn.particles, OP_N, T(-1, 0), T(1, 0));
-            else add(g, rhs1, bk.M2P[nn], OP_N, bn.multipoles, OP_N, T(-1, 0), T(1, 0));
+            if (!bn.Eliminated) add(g, rhs1, bk.P2P[nn], OP_N, bn.particles, OP_N, T(-1, 0), T(1, 0), 1);
+            else add(g, rhs1, bk.M2P[nn], OP_N, bn.multipoles, OP_N, T(-1, 0), T(1, 0), 1);
           }
           run(g);
           DMat<T> t = clone(rhs1);
@@ -1294,6 +1329,44 @@ class Solver : public SolverBase {
     s.factor_bytes += (double)luBase.size() * sizeof(T);
     return s;
   }
+  bool block(int level, int box, int which, int key, int& r, int& c, std::vector<double>& out) override {
+    if (level < 0 || level > nLevels || box < 0 || box >= nBoxes[level]) return false;
+    Box<T>& b = tree[level][box];
+    DMat<T>* m = nullptr;
+    auto find = [&](std::map<int, DMat<T>>& mp) -> DMat<T>* {
+      auto it = mp.find(key);
+      return it == mp.end() ? nullptr : &it->second;
+    };
+    switch (which) {
+      case 0: m = &b.L2P; break;
+      case 1: m = &b.P2M; break;
+      case 2: m = find(b.M2L); break;
+      case 3: m = find(b.P2P); break;
+      case 4: m = find(b.M2P); break;
+      case 5: m = find(b.P2L); break;
+      case 6: m = &b.L2M_f; break;
+      case 7: m = &b.particle_rhs; break;
+      case 8: m = &b.multipole_rhs; break;
+      case 9: m = &b.local_rhs; break;
+      case 10: m = &b.particles; break;
+      case 11: m = &b.multipoles; break;
+      case 12: m = &b.locals; break;
+      case 13: m = find(b.L2P_f); break;
+      default: return false;
+    }
+    if (!m) return false;
+    r = m->r;
+    c = m->c;
+    out.assign(2 * m->size(), 0.0);
+    if (m->size()) {
+      DMat<T> t = clone(*m);
+      std::vector<double> tmp(2 * m->size());
+      download(t, tmp.data());
+      out = tmp;
+      drop(t);
+    }
+    return true;
+  }
   void ranks(int level, std::vector<int>& out) override {
     out.clear();
     if (level < 0 || level > nLevels) return;
@@ -1319,6 +1392,14 @@ class Solver : public SolverBase {
   DMat<T> luBase;
   int* ipBase = nullptr;
   int NMb = 0, NLb = 0;
+  // RCS_DEBUG_STOP="level:box" stops the factorization after that box (inspection only)
+  int dbg_level = -1, dbg_box = -1;
+
+ public:
+  void read_debug_env() {
+    const char* e = std::getenv("RCS_DEBUG_STOP");
+    if (e) std::sscanf(e, "%d:%d", &dbg_level, &dbg_box);
+  }
 };
 
 template <>

@@ -55,6 +55,9 @@ class SolverBase {
   virtual void solve(const double* rhs, int nrhs, double* x, double* x_box) = 0;
   virtual SolverStats stats() const = 0;
   virtual void ranks(int level, std::vector<int>& out) = 0;
+  // Copy one stored block to the host as complex128 (which: 0 L2P, 1 P2M, 2 M2L[key],
+  // 3 P2P[key], 4 M2P[key], 5 P2L[key], 6 L2M_f). Returns false if absent.
+  virtual bool block(int level, int box, int which, int key, int& r, int& c, std::vector<double>& out) = 0;
 };
 
 std::unique_ptr<SolverBase> make_solver(std::shared_ptr<DeviceGeometry> g, const SolverParams& p, int precision,
