@@ -10,12 +10,15 @@ one solve, all on the GPU. Synthetic geometry; no files are read.
 Prints one JSON line (see the repository's benchmark contract).
 """
 import argparse
+import faulthandler
 import json
 import os
 import sys
 import time
 
 import numpy as np
+
+faulthandler.enable()  # a native crash prints the Python stack that led to it
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -88,10 +91,17 @@ def main():
     b = g.rhs()  # reference plane wave: x-polarised, phase vector (0, 0, 1) as coded at E:817-822
     R.set_profiling(True)
 
+    def log(msg):
+        print(f"[bench {args.config} N={g.N}] {msg}", file=sys.stderr, flush=True)
+
     def step():
+        log("assemble")
         s = R.AIFMM(g, bb, 64, tol, tol, 0, prec)
+        log("factorize")
         s.factorize()
+        log("solve")
         x = s.solve(b)
+        log("step done")
         return s, x
 
     for _ in range(args.warmup):

@@ -346,8 +346,8 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
     x.nchunks = (x.nl + ACA_CHUNK - 1) / ACA_CHUNK;
     int* dr = (int*)alloc(sizeof(int) * x.N1);
     int* dc = (int*)alloc(sizeof(int) * x.N2);
-    HIPCHK(hipMemcpyAsync(dr, P.rows.data(), sizeof(int) * x.N1, hipMemcpyHostToDevice, cx.st));
-    HIPCHK(hipMemcpyAsync(dc, P.cols.data(), sizeof(int) * x.N2, hipMemcpyHostToDevice, cx.st));
+    cx.upload(dr, P.rows.data(), sizeof(int) * x.N1);
+    cx.upload(dc, P.cols.data(), sizeof(int) * x.N2);
     x.rows = dr;
     x.cols = dc;
     x.U = (T*)cx.pool.alloc(sizeof(T) * (size_t)x.ns * x.cap);
@@ -367,7 +367,7 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
     live.push_back(b);
   }
   AcaBox<T>* d_boxes = (AcaBox<T>*)cx.pool.alloc(sizeof(AcaBox<T>) * nb);
-  HIPCHK(hipMemcpyAsync(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb, hipMemcpyHostToDevice, cx.st));
+  cx.upload(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb);
   struct Hdr { int rank, active, retry, cap; };
   std::vector<int> active = live;
   while (!active.empty()) {
@@ -405,7 +405,7 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
         grown = true;
       }
     }
-    if (grown) HIPCHK(hipMemcpyAsync(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb, hipMemcpyHostToDevice, cx.st));
+    if (grown) cx.upload(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb);
     active.swap(next);
   }
   for (int b : live) {

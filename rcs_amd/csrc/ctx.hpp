@@ -1,6 +1,7 @@
 // Execution context: one HIP stream, a caching HBM block pool and a staging ring for
 // launch parameter arrays (pinned host memory mirrored into HBM in stream order).
 #pragma once
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -171,18 +172,33 @@ struct Ctx {
 
   // Copy a small parameter array to HBM in stream order; returns the device address.
   void* push(const void* data, size_t bytes) {
-    bytes = (bytes + 255) & ~size_t(255);
-    if (bytes > cap) throw std::runtime_error("Ctx::push: parameter block larger than the staging ring");
-    if (off + bytes > cap) {
+    const size_t span = (bytes + 255) & ~size_t(255);  // keeps every block 256-byte aligned
+    if (span > cap) throw std::runtime_error("Ctx::push: parameter block larger than the staging ring");
+    if (off + span > cap) {
       HIPCHK(hipStreamSynchronize(st));  // every earlier copy out of the ring has completed
       off = 0;
     }
-    std::memcpy(h_stage + off, data, bytes);
+    std::memcpy(h_stage + off, data, bytes);  // exactly the caller's bytes, never past its end
     void* d = d_stage + off;
     HIPCHK(hipMemcpyAsync(d, h_stage + off, bytes, hipMemcpyHostToDevice, st));
-    off += bytes;
+    off += span;
+    pushed += span;
     return d;
   }
+  // Stream-ordered host -> HBM copy of a host buffer of any size into device memory the
+  // caller owns: staged through the pinned ring (no pageable-memory DMA), then copied
+  // device to device, so the ring space can be reused by later launches.
+  void upload(void* dst, const void* src, size_t bytes) {
+    const size_t chunk = cap / 4;
+    for (size_t o = 0; o < bytes; o += chunk) {
+      const size_t n = std::min(chunk, bytes - o);
+      void* d = push((const char*)src + o, n);
+      HIPCHK(hipMemcpyAsync((char*)dst + o, d, n, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  // Bytes pushed since the counter was last reset: a batch whose parameter blocks exceed
+  // the ring would overwrite its own earlier blocks before its launch.
+  size_t pushed = 0;
   void sync() { HIPCHK(hipStreamSynchronize(st)); }
   // Fetch n doubles of d_scalars (synchronizes).
   const double* fetch_scalars(size_t n) {

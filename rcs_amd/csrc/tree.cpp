@@ -194,10 +194,25 @@ class Solver : public SolverBase {
                std::vector<DMat<T>*>& outs) {
     std::vector<EntryTask<T>> tasks;
     int tiles = 0;
+    size_t base = cx.pushed;
+    auto flush = [&]() {
+      if (tasks.empty()) return;
+      double ne = 0;
+      for (auto& t : tasks) ne += (double)t.nr * t.nc;
+      const EntryTask<T>* d = (const EntryTask<T>*)cx.push(tasks.data(), sizeof(EntryTask<T>) * tasks.size());
+      int h = cx.prof.begin(cx.st);
+      launch_entries<T>(G->d, G->h.kind, d, (int)tasks.size(), tiles, cx.st);
+      cx.prof.end(cx.st, h, PROF_ENTRIES, ne);
+      tasks.clear();
+      tiles = 0;
+      base = cx.pushed;
+    };
     for (size_t i = 0; i < rows.size(); ++i) {
       int nr = (int)rows[i]->size(), nc = (int)cols[i]->size();
       put(*outs[i], newmat(nr, nc));
       if (nr == 0 || nc == 0) continue;
+      // the index lists of one launch must all stay live in the staging ring
+      if (cx.pushed - base > cx.cap / 4 || tiles > (1 << 26)) flush();
       EntryTask<T> t{};
       t.rows = upload_ints(*rows[i]);
       t.cols = upload_ints(*cols[i]);
@@ -208,17 +223,8 @@ class Solver : public SolverBase {
       t.tile0 = tiles;
       tiles += entry_tiles(nr, nc);
       tasks.push_back(t);
-      // keep one launch under the staging ring and int limits
-      if (tiles > (1 << 28)) throw std::runtime_error("entries: launch too large");
     }
-    if (!tasks.empty()) {
-      double ne = 0;
-      for (auto& t : tasks) ne += (double)t.nr * t.nc;
-      const EntryTask<T>* d = (const EntryTask<T>*)cx.push(tasks.data(), sizeof(EntryTask<T>) * tasks.size());
-      int h = cx.prof.begin(cx.st);
-      launch_entries<T>(G->d, G->h.kind, d, (int)tasks.size(), tiles, cx.st);
-      cx.prof.end(cx.st, h, PROF_ENTRIES, ne);
-    }
+    flush();
   }
 
   // ------------------------------------------------------------ tree set-up (T:52-331, 425-668)
@@ -351,13 +357,14 @@ class Solver : public SolverBase {
         int h = cx.prof.begin(cx.st);
         aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
         cx.prof.end(cx.st, h, PROF_ACA, 0.0);
+        trace("level %d incoming skeletons (%d boxes)", j, (int)which.size());
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
         std::vector<std::vector<int>> pivcols(nb);
         for (int k : which) {
           Box<T>& b = tree[j][k];
-          for (int r : res[k].row_bases) b.in_check.push_back(probs[k].rows[r]);
-          for (int c : res[k].col_bases) { b.in_charge.push_back(probs[k].cols[c]); pivcols[k].push_back(probs[k].cols[c]); }
+          for (int r : res[k].row_bases) b.in_check.push_back(probs[k].rows.at(r));
+          for (int c : res[k].col_bases) { b.in_charge.push_back(probs[k].cols.at(c)); pivcols[k].push_back(probs[k].cols.at(c)); }
           rr.push_back(&probs[k].rows);
           cc.push_back(&pivcols[k]);
           outs.push_back(&b.Ac);  // Ac = K(rows, pivot columns)
@@ -380,13 +387,14 @@ class Solver : public SolverBase {
         int h = cx.prof.begin(cx.st);
         aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
         cx.prof.end(cx.st, h, PROF_ACA, 0.0);
+        trace("level %d outgoing skeletons (%d boxes)", j, (int)which.size());
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
         std::vector<std::vector<int>> pivrows(nb);
         for (int k : which) {
           Box<T>& b = tree[j][k];
-          for (int r : res[k].row_bases) { b.out_check.push_back(probs[k].rows[r]); pivrows[k].push_back(probs[k].rows[r]); }
-          for (int c : res[k].col_bases) b.out_charge.push_back(probs[k].cols[c]);
+          for (int r : res[k].row_bases) { b.out_check.push_back(probs[k].rows.at(r)); pivrows[k].push_back(probs[k].rows.at(r)); }
+          for (int c : res[k].col_bases) b.out_charge.push_back(probs[k].cols.at(c));
           rr.push_back(&pivrows[k]);
           cc.push_back(&probs[k].cols);
           outs.push_back(&b.Ar);  // Ar = K(pivot rows, cols)
@@ -535,12 +543,15 @@ class Solver : public SolverBase {
   void assemble() override {
     double t0 = wall();
     setup();
+    trace("setup: N=%d nLevels=%d", N, nLevels);
     getNodes();
+    trace("skeletons done, startLevel=%d", startLevel);
     assemble_M2L();
     free_points();
     initialise_phase();
     initialise_P2P_Leaf_Level();
     cx.sync();
+    trace("assembly done");
     stats_.t_assemble = wall() - t0;
   }
 
@@ -1093,6 +1104,7 @@ class Solver : public SolverBase {
         }
         (void)tc;
         tree[j][k].Eliminated = true;
+        if (k % 16 == 0) trace("eliminate level %d box %d", j, k);
         eliminate_x(j, k);
         eliminate_z(j, k);
         if (j == dbg_level && k == dbg_box) {  // debugging aid: stop after this box
@@ -1101,8 +1113,10 @@ class Solver : public SolverBase {
         }
       }
     }
+    trace("base level: %d boxes", nBoxes[startLevel]);
     factor_base_level();
     cx.sync();
+    trace("factorization done");
     stats_.t_factor = wall() - t0;
     stats_.peak_bytes = cx.pool.peak();
   }
@@ -1158,7 +1172,7 @@ class Solver : public SolverBase {
       DMat<cpx<double>> B64;
       B64.r = N; B64.c = nrhs; B64.ld = N;
       B64.p = (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs);
-      HIPCHK(hipMemcpyAsync(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs, hipMemcpyHostToDevice, cx.st));
+      cx.upload(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs);
       to_T(B, B64);
       cx.pool.free(B64.p);
     }
@@ -1394,6 +1408,15 @@ class Solver : public SolverBase {
   int NMb = 0, NLb = 0;
   // RCS_DEBUG_STOP="level:box" stops the factorization after that box (inspection only)
   int dbg_level = -1, dbg_box = -1;
+  // RCS_VERBOSE=1 prints one line per phase to stderr (flushed)
+  bool verbose = std::getenv("RCS_VERBOSE") != nullptr;
+  void trace(const char* fmt, int a = 0, int b = 0) {
+    if (!verbose) return;
+    std::fprintf(stderr, "[rcs] ");
+    std::fprintf(stderr, fmt, a, b);
+    std::fprintf(stderr, " (pool %.2f GiB)\n", cx.pool.in_use() / 1073741824.0);
+    std::fflush(stderr);
+  }
 
  public:
   void read_debug_env() {
