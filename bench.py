@@ -155,10 +155,13 @@ def main():
         "algorithmic_flops": flops,
         "n_rrqr": st["n_rrqr"],
         "peak_hbm_gib": st["peak_bytes"] / 2 ** 30,
-        "roofline": {"bound": "mfma", "kernel": "gemm_grouped_kernel (Schur fill-in updates)",
+        "roofline": {"bound": "mfma", "kernel": "gemm_grouped_kernel<cpx<double>, 1> (Schur fill-in updates)",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "launches": schur["launches"], "avg_launch_ms": schur["ms"] / max(schur["launches"], 1)},
+                     "launches": schur["launches"], "avg_launch_ms": schur["ms"] / max(schur["launches"], 1),
+                     "flops_per_launch": schur["flops"] / max(schur["launches"], 1)},
+        "phases_event_ms": {k: {"ms": round(v["ms"] / args.steps, 3), "launches": v["launches"] // args.steps,
+                                "work": v["flops"] / args.steps} for k, v in prof.items()},
         "rcs_theta0_dbsm": float(10 * np.log10(rcs[0])),
     }
     if not args.no_cpu:

@@ -56,8 +56,12 @@ DMat<T> upload(Ctx& cx, const double* h, int r, int c, int ldh);
 template <>
 DMat<cpx<double>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
   DMat<cpx<double>> m = dalloc<cpx<double>>(cx, r, c);
-  HIPCHK(hipMemcpy2DAsync(m.p, sizeof(cpx<double>) * m.ld, h, sizeof(cpx<double>) * ldh, sizeof(cpx<double>) * r, c,
-                          hipMemcpyHostToDevice, cx.st));
+  if (ldh == m.ld) {
+    cx.upload(m.p, h, sizeof(cpx<double>) * (size_t)r * c);
+  } else {
+    for (int j = 0; j < c; ++j)
+      cx.upload(m.p + (size_t)j * m.ld, h + 2 * (size_t)j * ldh, sizeof(cpx<double>) * (size_t)r);
+  }
   return m;
 }
 template <>

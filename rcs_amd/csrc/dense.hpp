@@ -39,8 +39,12 @@ struct GemmOp {
 
 // C_g = alpha*sum_s op(A_s)*op(B_s) + beta*C_g for every output block of the group, in
 // one launch. Two output blocks of one group must not share C (checked on the host).
+// role 1 marks the Schur-complement updates of the elimination (a separate kernel
+// instantiation, so traces attribute them); 0 is every other product. prof_tag >= 0
+// times the kernel alone with HIP events under that profiler tag (algorithmic flops
+// 8*m*n*k per product).
 template <typename T>
-void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx);
+void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role = 0, int prof_tag = -1);
 
 // In-place LU with partial pivoting (first maximal |a| in the column), ipiv 0-based
 // absolute row swapped with each row (LAPACK style). ipiv is a device array of n ints.
@@ -84,7 +88,7 @@ void cast_c128_to_c64(DMat<cpx<float>> dst, DMat<cpx<double>> src, hipStream_t s
 void cast_c64_to_c128(DMat<cpx<double>> dst, DMat<cpx<float>> src, hipStream_t st);
 
 // Two-partition column-pivoted Householder QR (RRQR::colPivHouseholderQRWithPartition,
-// src/rrqr.cpp:69-182). A (m x n) is overwritten. Q (m x min(m,n)) receives Q_trunc.
+// src/rrqr.cpp:69-182). A (m x n) is read only. Q (m x min(m,n)) receives Q_trunc.
 struct RRQRInfo {
   int rank, rank1, rank2;
   double maxPivot;
@@ -92,6 +96,17 @@ struct RRQRInfo {
 template <typename T>
 RRQRInfo rrqr(DMat<T> A, int partition, double tol1, double tol2, double inputPivot, int fillInRank, DMat<T> Q,
               Ctx& cx);
-size_t rrqr_work_bytes(int m, int n, int elem_bytes);
+// Several independent RRQRs with the same tolerances through one launch per pivot step.
+template <typename T>
+struct RRQRTask {
+  DMat<T> A;         // input, read only
+  int partition = 0;
+  double inputPivot = 0;
+  int fillInRank = 0;
+  DMat<T> Q;         // m x (>= rank) output
+  RRQRInfo info{0, 0, 0, 0.0};
+};
+template <typename T>
+void rrqr_batched(std::vector<RRQRTask<T>>& tasks, double tol1, double tol2, Ctx& cx);
 
 }  // namespace rcs

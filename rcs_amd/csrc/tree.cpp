@@ -133,11 +133,7 @@ class Solver : public SolverBase {
     g.ops.push_back(o);
   }
   void run(Group& g, int tag = -1) {
-    double fl = 0;
-    for (auto& o : g.ops) fl += 8.0 * o.m * (double)o.n * o.k;
-    int h = tag >= 0 ? cx.prof.begin(cx.st) : -1;
-    gemm_grouped(g.ops, cx);
-    if (tag >= 0) cx.prof.end(cx.st, h, tag, fl);
+    gemm_grouped(g.ops, cx, tag == PROF_SCHUR ? 1 : 0, tag);
     g.ops.clear();
   }
   DMat<T> mul(DMat<T> A, int opA, DMat<T> B, int opB, T alpha = T(1, 0)) {
@@ -814,25 +810,37 @@ class Solver : public SolverBase {
     DMat<T> Q;
     double maxPivot;
   };
-  QR do_rrqr(DMat<T> A, int partition, double inputPivot, int j) {
-    DMat<T> W = clone(A);
-    DMat<T> Qb = newmat(A.r, std::max(1, std::min(A.r, A.c)));
+  // Independent RRQRs (same level tolerances and fill-in cap) through shared launches.
+  std::vector<QR> do_rrqr_batch(const std::vector<DMat<T>>& As, const std::vector<int>& parts,
+                                const std::vector<double>& pivots, int j) {
+    const size_t nb = As.size();
+    std::vector<RRQRTask<T>> tk(nb);
+    for (size_t i = 0; i < nb; ++i) {
+      tk[i].A = As[i];
+      tk[i].partition = parts[i];
+      tk[i].inputPivot = pivots[i];
+      tk[i].fillInRank = rankVec[j];
+      tk[i].Q = newmat(As[i].r, std::max(1, std::min(As[i].r, As[i].c)));
+    }
     double t0 = wall();
     int h = cx.prof.begin(cx.st);
-    RRQRInfo inf = rrqr(W, partition, std::pow(10.0, -TOL_POW), std::pow(10.0, -fill_in_TOL_POW), inputPivot,
-                        rankVec[j], Qb, cx);
-    cx.prof.end(cx.st, h, PROF_RRQR, 0.0);
+    rrqr_batched(tk, std::pow(10.0, -TOL_POW), std::pow(10.0, -fill_in_TOL_POW), cx);
+    double fl = 0;
+    std::vector<QR> out(nb);
+    for (size_t i = 0; i < nb; ++i) {
+      const RRQRInfo& inf = tk[i].info;
+      double r = inf.rank, m = As[i].r, n = As[i].c;
+      fl += 8.0 * (2.0 * m * n * r - r * r * (m + n) + 2.0 * r * r * r / 3.0) + 8.0 * (2.0 * m * r * r - 2.0 * r * r * r / 3.0);
+      out[i].Q = newmat(As[i].r, inf.rank);
+      if (inf.rank > 0) copy2d(out[i].Q, tk[i].Q.cols(0, inf.rank), cx.st);
+      out[i].maxPivot = inf.maxPivot;
+      drop(tk[i].Q);
+    }
+    cx.prof.end(cx.st, h, PROF_RRQR, fl);
     stats_.t_compress += wall() - t0;
-    stats_.n_rrqr++;
-    double r = inf.rank, m = A.r, n = A.c;
-    stats_.qr_flops += 8.0 * (2.0 * m * n * r - r * r * (m + n) + 2.0 * r * r * r / 3.0) + 8.0 * (2.0 * m * r * r - 2.0 * r * r * r / 3.0);
-    QR q;
-    q.Q = newmat(A.r, inf.rank);
-    if (inf.rank > 0) copy2d(q.Q, Qb.cols(0, inf.rank), cx.st);
-    q.maxPivot = inf.maxPivot;
-    drop(W);
-    drop(Qb);
-    return q;
+    stats_.n_rrqr += (long long)nb;
+    stats_.qr_flops += fl;
+    return out;
   }
   DMat<T> hcat2(DMat<T> A, int opA, DMat<T> B, int opB) {  // [op(A) | op(B)]
     int ra = opA == OP_N ? A.r : A.c, ca = opA == OP_N ? A.c : A.r;
@@ -933,45 +941,80 @@ class Solver : public SolverBase {
     drop(Rm);
     drop(Vb);
   }
-  // T:1573-1656
-  void compress_P2P_myRRQR(int j, int k, int b, DMat<T> oldL2P, DMat<T> oldP2M, DMat<T> F, DMat<T>& newL2P,
-                           DMat<T>& newP2Mt, DMat<T>& M2Lkb) {
-    DMat<T> Acol = hcat2(oldL2P, OP_N, F, OP_N);
-    if (std::getenv("RCS_DEBUG_LOG")) {
-      double nf = norms({F})[0];
-      std::fprintf(stderr, "P2P j%d k%d b%d A %dx%d part %d ip %.6e nF %.6e\n", j, k, b, Acol.r, Acol.c, oldL2P.c,
-                   tree[j][k].maxPivot_L2P, nf);
+  // compress_P2P_myRRQR (T:1573-1656) for both directions of compress_P2P_Two_Ways:
+  // d = 0 is (k, b), d = 1 is (b, k). The two directions read disjoint state (bases,
+  // fill-ins, pivots and M2L blocks of opposite roles), so their RRQRs share launches.
+  // Outputs per direction: newL2P of the row box, newP2M^H of the column box, new M2L.
+  void compress_P2P_both(int j, int k, int b, DMat<T> newL2P[2], DMat<T> newP2Mt[2], DMat<T> M2L[2]) {
+    const int ra[2] = {k, b}, ca[2] = {b, k};
+    std::vector<DMat<T>> A(2);
+    std::vector<int> part(2);
+    std::vector<double> piv(2);
+    for (int d = 0; d < 2; ++d) {
+      Box<T>& br = tree[j][ra[d]];
+      A[d] = hcat2(br.L2P, OP_N, br.P2P[ca[d]], OP_N);
+      part[d] = br.L2P.c;
+      piv[d] = br.maxPivot_L2P;
     }
-    QR q1 = do_rrqr(Acol, oldL2P.c, tree[j][k].maxPivot_L2P, j);
-    if (std::getenv("RCS_DEBUG_LOG")) std::fprintf(stderr, "  r1 %d mp %.6e\n", q1.Q.c, q1.maxPivot);
-    drop(Acol);
-    tree[j][k].maxPivot_L2P = std::max(tree[j][k].maxPivot_L2P, q1.maxPivot);
-    newL2P = q1.Q;
-    DMat<T> Rk = mul(newL2P, OP_C, oldL2P, OP_N);
-    DMat<T> Rkp = mul(newL2P, OP_C, F, OP_N);
-    DMat<T> Arow = hcat2(oldP2M, OP_C, Rkp, OP_C);
-    QR q2 = do_rrqr(Arow, oldP2M.r, tree[j][b].maxPivot_P2M, j);
-    if (std::getenv("RCS_DEBUG_LOG"))
-      std::fprintf(stderr, "  r2 %d mp %.6e ip %.6e A %dx%d\n", q2.Q.c, q2.maxPivot, tree[j][b].maxPivot_P2M, Arow.r, Arow.c);
-    drop(Arow);
-    tree[j][b].maxPivot_P2M = std::max(tree[j][b].maxPivot_P2M, q2.maxPivot);
-    newP2Mt = q2.Q;
-    DMat<T> Rn = mul(newP2Mt, OP_C, oldP2M, OP_C);
-    DMat<T> Rnp = mul(newP2Mt, OP_C, Rkp, OP_C);
-    DMat<T>& M = tree[j][k].M2L[b];
-    if (M.size() > 0) {
-      DMat<T> t = mul(Rk, OP_N, M, OP_N);
-      M2Lkb = newmat(Rnp.c, Rnp.r);
-      adjoint2d(M2Lkb, Rnp, cx.st);
+    std::vector<QR> q1 = do_rrqr_batch(A, part, piv, j);
+    DMat<T> Rk[2], Rkp[2];
+    for (int d = 0; d < 2; ++d) {
+      Box<T>& br = tree[j][ra[d]];
+      drop(A[d]);
+      br.maxPivot_L2P = std::max(br.maxPivot_L2P, q1[d].maxPivot);
+      newL2P[d] = q1[d].Q;
+      Rk[d] = newmat(newL2P[d].c, br.L2P.c);
+      Rkp[d] = newmat(newL2P[d].c, br.P2P[ca[d]].c);
+    }
+    {
       Group g;
-      add(g, M2Lkb, t, OP_N, Rn, OP_C, T(1, 0), T(1, 0));
+      for (int d = 0; d < 2; ++d) {
+        Box<T>& br = tree[j][ra[d]];
+        add(g, Rk[d], newL2P[d], OP_C, br.L2P, OP_N, T(1, 0), T(0, 0));
+        add(g, Rkp[d], newL2P[d], OP_C, br.P2P[ca[d]], OP_N, T(1, 0), T(0, 0));
+      }
       run(g);
-      drop(t);
-    } else {
-      M2Lkb = newmat(Rnp.c, Rnp.r);
-      adjoint2d(M2Lkb, Rnp, cx.st);
     }
-    drop(Rk); drop(Rkp); drop(Rn); drop(Rnp);
+    for (int d = 0; d < 2; ++d) {
+      Box<T>& bc = tree[j][ca[d]];
+      A[d] = hcat2(bc.P2M, OP_C, Rkp[d], OP_C);
+      part[d] = bc.P2M.r;
+      piv[d] = bc.maxPivot_P2M;
+    }
+    std::vector<QR> q2 = do_rrqr_batch(A, part, piv, j);
+    DMat<T> Rn[2], Rnp[2], t[2];
+    for (int d = 0; d < 2; ++d) {
+      Box<T>& bc = tree[j][ca[d]];
+      drop(A[d]);
+      bc.maxPivot_P2M = std::max(bc.maxPivot_P2M, q2[d].maxPivot);
+      newP2Mt[d] = q2[d].Q;
+      Rn[d] = newmat(newP2Mt[d].c, bc.P2M.r);
+      Rnp[d] = newmat(newP2Mt[d].c, Rkp[d].r);
+    }
+    {
+      Group g;
+      for (int d = 0; d < 2; ++d) {
+        Box<T>& bc = tree[j][ca[d]];
+        add(g, Rn[d], newP2Mt[d], OP_C, bc.P2M, OP_C, T(1, 0), T(0, 0));
+        add(g, Rnp[d], newP2Mt[d], OP_C, Rkp[d], OP_C, T(1, 0), T(0, 0));
+        DMat<T>& M = tree[j][ra[d]].M2L[ca[d]];
+        if (M.size() > 0) {
+          t[d] = newmat(Rk[d].r, M.c);
+          add(g, t[d], Rk[d], OP_N, M, OP_N, T(1, 0), T(0, 0));
+        }
+      }
+      run(g);
+    }
+    {
+      Group g;
+      for (int d = 0; d < 2; ++d) {  // M2L = Rnp^H + (Rk M) Rn^H
+        M2L[d] = newmat(Rnp[d].c, Rnp[d].r);
+        adjoint2d(M2L[d], Rnp[d], cx.st);
+        if (t[d].size() > 0) add(g, M2L[d], t[d], OP_N, Rn[d], OP_C, T(1, 0), T(1, 0));
+      }
+      run(g);
+    }
+    for (int d = 0; d < 2; ++d) { drop(Rk[d]); drop(Rkp[d]); drop(Rn[d]); drop(Rnp[d]); drop(t[d]); }
   }
   void square_up(DMat<T> L2P, DMat<T> P2Mt, DMat<T>& L2Pm, DMat<T>& P2Mtm) {  // T:2168-2183
     if (L2P.c < P2Mt.c) { P2Mtm = P2Mt.cols(0, L2P.c); L2Pm = L2P; }
@@ -980,9 +1023,10 @@ class Solver : public SolverBase {
   void compress_P2P_Two_Ways(int j, int k, int b) {  // T:2116-2306
     if (tree[j][b].M2L[k].size() == 0 || tree[j][k].M2L[b].size() == 0) return;
     stats_.n_compress_p2p++;
-    DMat<T> nL2Pk, nP2Mtb, M2Lkb, nL2Pb, nP2Mtk, M2Lbk;
-    compress_P2P_myRRQR(j, k, b, tree[j][k].L2P, tree[j][b].P2M, tree[j][k].P2P[b], nL2Pk, nP2Mtb, M2Lkb);
-    compress_P2P_myRRQR(j, b, k, tree[j][b].L2P, tree[j][k].P2M, tree[j][b].P2P[k], nL2Pb, nP2Mtk, M2Lbk);
+    DMat<T> nL2P[2], nP2Mt[2], nM2L[2];
+    compress_P2P_both(j, k, b, nL2P, nP2Mt, nM2L);
+    DMat<T> nL2Pk = nL2P[0], nP2Mtb = nP2Mt[0], M2Lkb = nM2L[0];
+    DMat<T> nL2Pb = nL2P[1], nP2Mtk = nP2Mt[1], M2Lbk = nM2L[1];
     DMat<T> L2Pk_m, P2Mtk_m, L2Pb_m, P2Mtb_m;
     square_up(nL2Pk, nP2Mtk, L2Pk_m, P2Mtk_m);
     square_up(nL2Pb, nP2Mtb, L2Pb_m, P2Mtb_m);
@@ -1023,9 +1067,14 @@ class Solver : public SolverBase {
     stats_.n_compress_p2l++;
     Box<T>& bk = tree[j][k];
     Box<T>& bb = tree[j][b];
+    // the column and row RRQRs read disjoint inputs: one batched launch sequence
     DMat<T> Acol = hcat2(bk.L2P, OP_N, bk.M2P[b], OP_N);
-    QR q1 = do_rrqr(Acol, bk.L2P.c, bk.maxPivot_L2P, j);
+    DMat<T> Arow = hcat2(bk.P2M, OP_C, bb.P2L[k], OP_C);
+    std::vector<QR> qq = do_rrqr_batch({Acol, Arow}, {bk.L2P.c, bk.P2M.r}, {bk.maxPivot_L2P, bk.maxPivot_P2M}, j);
     drop(Acol);
+    drop(Arow);
+    const QR& q1 = qq[0];
+    const QR& q2 = qq[1];
     DMat<T> nL2P = q1.Q;
     bk.maxPivot_L2P = std::max(bk.maxPivot_L2P, q1.maxPivot);
     DMat<T> Rk = mul(nL2P, OP_C, bk.L2P, OP_N);
@@ -1035,9 +1084,6 @@ class Solver : public SolverBase {
       add(g, M2Lkb, Rk, OP_N, bk.M2L[b], OP_N, T(1, 0), T(1, 0));
       run(g);
     }
-    DMat<T> Arow = hcat2(bk.P2M, OP_C, bb.P2L[k], OP_C);
-    QR q2 = do_rrqr(Arow, bk.P2M.r, bk.maxPivot_P2M, j);
-    drop(Arow);
     DMat<T> nP2Mt = q2.Q;
     bk.maxPivot_P2M = std::max(bk.maxPivot_P2M, q2.maxPivot);
     DMat<T> Rn = mul(nP2Mt, OP_C, bk.P2M, OP_C);
