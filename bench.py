@@ -54,7 +54,7 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops):
     est = wall * gpu_flops / max(flops, 1.0)
     return {
         "value": 1.0 / est,
-        "unit": "factorize+solve/s",
+        "unit": "assemble+factorize+solve/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
         "kind": "port",
         "sample": (f"oracle (FP64 CPU restatement) full assemble+factor+solve of the level-{level_cpu} sphere "
@@ -66,6 +66,40 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops):
     }
 
 
+def measured_traffic(config):
+    """HBM bytes per launch of the Schur update kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size); None when no
+    summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"schur_gemm_traffic_{config}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def copy_calibration(mib):
+    """Device-to-device copies of a known size through the HIP runtime's copy kernel: the
+    reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    reps = 3
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes_per_copy": n, "copies": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +108,12 @@ def main():
     ap.add_argument("--config", default="efie-sphere-30720", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -157,7 +196,8 @@ def main():
         "peak_hbm_gib": st["peak_bytes"] / 2 ** 30,
         "roofline": {"bound": "mfma", "kernel": "gemm_grouped_kernel<cpx<double>, 1> (Schur fill-in updates)",
                      "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic(args.config),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
                      "launches": schur["launches"], "avg_launch_ms": schur["ms"] / max(schur["launches"], 1),
                      "flops_per_launch": schur["flops"] / max(schur["launches"], 1)},
         "phases_event_ms": {k: {"ms": round(v["ms"] / args.steps, 3), "launches": v["launches"] // args.steps,

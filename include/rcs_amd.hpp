@@ -1,0 +1,289 @@
+/*
+ * C++ host interface of the MI355X AIFMM library, mirroring the reference's classes
+ * (vaishna77/RCS) so that a driver written against them keeps its shape:
+ *
+ *   kernel            include/kernel.hpp:13-47, src/kernel.cpp:11-73
+ *   EFIEMatrix        include/EFIEMatrix.hpp:29-118 (mesh files, N, getRhs, RCS)
+ *   ElectrostaticsMatrix include/ElectrostaticsMatrix.hpp:8-40
+ *   LowRank<K>        include/ACA.hpp:7-30 (ACA_only_nodes2)
+ *   RRQR              include/rrqr.hpp:5-27
+ *   AIFMM<K>          include/AIFMM.hpp:11-36 (factorize, solve, getPhi)
+ *
+ * Header only, over the C ABI of rcs_amd.h (link with -lrcs_amd). Matrices are
+ * column-major complex (rcs::Mat); the reference's Eigen Mat/Vec map onto data()/rows()/
+ * cols() one to one. Differences from the reference, on purpose: failures throw
+ * std::runtime_error with the library's message (the reference has no error path);
+ * freq is a constructor argument (the reference compiles it in, include/includes.hpp:58);
+ * single entries (getMatrixEntry) are served by the GPU one at a time and are meant
+ * for checks, the batched getMatrix/getRow/getCol for real use.
+ */
+#ifndef RCS_AMD_HPP
+#define RCS_AMD_HPP
+
+#include <algorithm>
+#include <complex>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rcs_amd.h"
+
+namespace rcs {
+
+using cdouble = std::complex<double>;
+using Vec = std::vector<cdouble>;
+
+// Column-major complex matrix (the role of Eigen::MatrixXcd in the reference).
+struct Mat {
+  int rows_ = 0, cols_ = 0;
+  std::vector<cdouble> v;
+  Mat() = default;
+  Mat(int r, int c) : rows_(r), cols_(c), v((size_t)r * c) {}
+  int rows() const { return rows_; }
+  int cols() const { return cols_; }
+  cdouble& operator()(int i, int j) { return v[(size_t)i + (size_t)j * rows_]; }
+  const cdouble& operator()(int i, int j) const { return v[(size_t)i + (size_t)j * rows_]; }
+  double* data() { return reinterpret_cast<double*>(v.data()); }
+  const double* data() const { return reinterpret_cast<const double*>(v.data()); }
+};
+
+inline void check(int rc) {
+  if (rc != 0) throw std::runtime_error(std::string("rcs_amd: ") + rcs_last_error());
+}
+
+struct Node {
+  double x = 0, y = 0, z = 0;
+};
+
+// include/includes.hpp:73-84
+struct inputsToKernelClass {
+  std::string nodeListFilename;
+  std::string patchListFilename;
+  int n = 0;
+  double L = 0;
+};
+struct inputsToSolverClass {
+  double xmin = 0, xmax = 0, ymin = 0, ymax = 0, zmin = 0, zmax = 0;
+  int leafSize = 64, tol_pow = 6, nLevelsInput = 0, fillin_rank = 0, fillin_tol = 6;
+  std::vector<double> locations;  // N x 3, row by row (the rows of the reference's locations matrix)
+};
+
+// Reference mesh files: "x y z" per node line, "i j k" (1-based) per patch line
+// (E:31-44, E:72-95).
+inline void readMeshFiles(const std::string& nodes, const std::string& patches, std::vector<double>& xyz,
+                          std::vector<int>& tris) {
+  std::ifstream fn(nodes), fp(patches);
+  if (!fn || !fp) throw std::runtime_error("rcs_amd: cannot open mesh files");
+  double a, b, c;
+  while (fn >> a >> b >> c) { xyz.push_back(a); xyz.push_back(b); xyz.push_back(c); }
+  int i, j, k;
+  while (fp >> i >> j >> k) { tris.push_back(i); tris.push_back(j); tris.push_back(k); }
+}
+
+// kernel (include/kernel.hpp): batched entry generation on the GPU.
+class kernel {
+ public:
+  int N = 0;
+  virtual ~kernel() { if (mesh_) rcs_mesh_destroy(mesh_); }
+  kernel(const kernel&) = delete;
+  kernel& operator=(const kernel&) = delete;
+
+  virtual cdouble getMatrixEntry(const unsigned i, const unsigned j) {
+    int r = (int)i, c = (int)j;
+    double out[2];
+    check(rcs_entries(mesh_, RCS_PREC_C128, &r, 1, &c, 1, out));
+    return {out[0], out[1]};
+  }
+  Mat getMatrix(std::vector<int> row_indices, std::vector<int> col_indices) {  // src/kernel.cpp:49-61
+    Mat m((int)row_indices.size(), (int)col_indices.size());
+    if (m.v.empty()) return m;
+    check(rcs_entries(mesh_, RCS_PREC_C128, row_indices.data(), m.rows(), col_indices.data(), m.cols(), m.data()));
+    return m;
+  }
+  Mat getMatrix(int row_start_index, int col_start_index, int row_end_index, int col_end_index) {
+    std::vector<int> r, c;  // src/kernel.cpp:63-73 (absolute indices, valid from 0 as there)
+    for (int i = row_start_index; i < row_end_index; ++i) r.push_back(i);
+    for (int j = col_start_index; j < col_end_index; ++j) c.push_back(j);
+    return getMatrix(r, c);
+  }
+  void getMatrix(Mat& mat, int n) { mat = getMatrix(0, 0, n, n); }
+  Vec getRow(const int j, std::vector<int> col_indices) {  // src/kernel.cpp:11-19
+    Mat m = getMatrix(std::vector<int>{j}, col_indices);
+    return Vec(m.v.begin(), m.v.end());
+  }
+  Vec getCol(const int k, std::vector<int> row_indices) {  // src/kernel.cpp:21-29
+    Mat m = getMatrix(row_indices, std::vector<int>{k});
+    return Vec(m.v.begin(), m.v.end());
+  }
+  Vec getRow(const int n, const int k) {  // row k over all n columns
+    std::vector<int> c(n);
+    for (int i = 0; i < n; ++i) c[i] = i;
+    return getRow(k, c);
+  }
+  Vec getCol(const int n, const int k) {  // column k over all n rows
+    std::vector<int> r(n);
+    for (int i = 0; i < n; ++i) r[i] = i;
+    return getCol(k, r);
+  }
+  rcs_mesh_t handle() const { return mesh_; }
+  double frequency() const { return freq_; }
+
+ protected:
+  kernel() = default;
+  void create(const std::vector<double>& xyz, const std::vector<int>& tris, int kind, double freq) {
+    freq_ = freq;
+    check(rcs_mesh_create(xyz.data(), (int)xyz.size() / 3, tris.data(), (int)tris.size() / 3, nullptr, kind, freq,
+                          &mesh_));
+    N = rcs_mesh_size(mesh_);
+  }
+  rcs_mesh_t mesh_ = nullptr;
+  double freq_ = 0;
+};
+
+// EFIEMatrix (include/EFIEMatrix.hpp:29-118): RWG EFIE operator of a closed PEC mesh.
+class EFIEMatrix : public kernel {
+ public:
+  EFIEMatrix(inputsToKernelClass in, double freq) {
+    std::vector<double> xyz;
+    std::vector<int> tris;
+    readMeshFiles(in.nodeListFilename, in.patchListFilename, xyz, tris);
+    create(xyz, tris, RCS_KERNEL_EFIE, freq);
+  }
+  EFIEMatrix(const std::vector<double>& xyz, const std::vector<int>& tris, double freq) {
+    create(xyz, tris, RCS_KERNEL_EFIE, freq);
+  }
+  // getRhs(Node& Einc, VectorXcd& rhs) (E:785-829); kvec defaults to the reference's
+  // hard-coded phase vector (0, 0, 1)
+  void getRhs(Node& Einc, Vec& rhs, Node kvec = Node{0, 0, 1}) {
+    rhs.assign(N, cdouble());
+    const double E[3] = {Einc.x, Einc.y, Einc.z}, k[3] = {kvec.x, kvec.y, kvec.z};
+    check(rcs_rhs(mesh_, E, k, reinterpret_cast<double*>(rhs.data())));
+  }
+  // getJAtCentroids + getRCS_OnePointQuadrature (E:916-930, E:1100-1130), m^2
+  std::vector<double> getRCS(const Vec& coefficients, const std::vector<double>& theta,
+                             const std::vector<double>& phi) {
+    std::vector<double> out(theta.size());
+    check(rcs_rcs(mesh_, reinterpret_cast<const double*>(coefficients.data()), theta.data(), phi.data(),
+                  (int)theta.size(), out.data()));
+    return out;
+  }
+  std::vector<double> locations() {
+    std::vector<double> xyz(3 * (size_t)N);
+    check(rcs_mesh_locations(mesh_, xyz.data()));
+    return xyz;
+  }
+};
+
+// ElectrostaticsMatrix (include/ElectrostaticsMatrix.hpp:8-40): piecewise-constant
+// single layer C0 * int 1/R at patch centroids.
+class ElectrostaticsMatrix : public kernel {
+ public:
+  explicit ElectrostaticsMatrix(inputsToKernelClass in) {
+    std::vector<double> xyz;
+    std::vector<int> tris;
+    readMeshFiles(in.nodeListFilename, in.patchListFilename, xyz, tris);
+    create(xyz, tris, RCS_KERNEL_LAPLACE, 1.0e9);
+  }
+  ElectrostaticsMatrix(const std::vector<double>& xyz, const std::vector<int>& tris) {
+    create(xyz, tris, RCS_KERNEL_LAPLACE, 1.0e9);
+  }
+  std::vector<double> locations() {
+    std::vector<double> xyz(3 * (size_t)N);
+    check(rcs_mesh_locations(mesh_, xyz.data()));
+    return xyz;
+  }
+};
+
+// LowRank<K> (include/ACA.hpp): ACA_only_nodes2 (src/ACA.cpp:639-859). row_bases and
+// col_bases are positions within row_indices / col_indices, as in the reference; Ac and
+// Ar are the original kernel columns and rows at those pivots.
+template <typename kerneltype>
+class LowRank {
+ public:
+  kerneltype* K;
+  double tolerance_or_rank;
+  std::vector<int> row_indices, col_indices;
+  LowRank(kerneltype* K_, double tol, std::vector<int>& rows, std::vector<int>& cols)
+      : K(K_), tolerance_or_rank(tol), row_indices(rows), col_indices(cols) {}
+  void ACA_only_nodes2(std::vector<int>& row_bases, std::vector<int>& col_bases, int& computed_rank, Mat& Ac,
+                       Mat& Ar) {
+    const int nr = (int)row_indices.size(), nc = (int)col_indices.size();
+    const int mn = std::min(nr, nc);
+    row_bases.assign(std::max(mn, 1), 0);
+    col_bases.assign(std::max(mn, 1), 0);
+    computed_rank = 0;
+    if (mn > 0)
+      check(rcs_aca(K->handle(), RCS_PREC_C128, tolerance_or_rank, row_indices.data(), nr, col_indices.data(), nc,
+                    &computed_rank, row_bases.data(), col_bases.data()));
+    row_bases.resize(computed_rank);
+    col_bases.resize(computed_rank);
+    std::vector<int> pr, pc;
+    for (int r : row_bases) pr.push_back(row_indices[r]);
+    for (int c : col_bases) pc.push_back(col_indices[c]);
+    Ac = K->getMatrix(row_indices, pc);
+    Ar = K->getMatrix(pr, col_indices);
+  }
+};
+
+// RRQR (include/rrqr.hpp): the partitioned column-pivoted Householder QR of
+// src/rrqr.cpp:69-182. Produces Q_trunc and maxPivot (what the tree uses).
+class RRQR {
+ public:
+  Mat A;
+  int fillInRank, partition, flag;
+  double inputPivot, maxPivot = 0;
+  int TOL_POW1, TOL_POW2;
+  Mat Q_trunc;
+  int rank = 0;
+  RRQR(Mat& A_, int tol_pow1, int tol_pow2, int partition_, double inputPivot_, int flag_, int fillInRank_)
+      : A(A_), fillInRank(fillInRank_), partition(partition_), flag(flag_), inputPivot(inputPivot_),
+        TOL_POW1(tol_pow1), TOL_POW2(tol_pow2) {}
+  void colPivHouseholderQRWithPartition() {
+    const int m = A.rows(), n = A.cols();
+    Mat Q(m, std::max(1, std::min(m, n)));
+    if (m > 0 && n > 0)
+      check(rcs_rrqr(RCS_PREC_C128, A.data(), m, n, TOL_POW1, TOL_POW2, partition, inputPivot, fillInRank, Q.data(),
+                     &rank, &maxPivot));
+    Q_trunc = Mat(m, rank);
+    for (int j = 0; j < rank; ++j)
+      for (int i = 0; i < m; ++i) Q_trunc(i, j) = Q(i, j);
+  }
+};
+
+// AIFMM<K> (include/AIFMM.hpp): construction assembles on the GPU, factorize() runs the
+// elimination, solve() returns the solution in box order (src/AIFMM.cpp:103-110) and
+// getPhi() the solution in the input order (src/AIFMM.cpp:135-147) of the last solve.
+template <typename kerneltype>
+class AIFMM {
+ public:
+  AIFMM(inputsToSolverClass in, kerneltype* mykernel, int precision = RCS_PREC_C128) : K(mykernel) {
+    const double bbox[6] = {in.xmin, in.xmax, in.ymin, in.ymax, in.zmin, in.zmax};
+    check(rcs_solver_create(mykernel->handle(), bbox, in.leafSize, in.tol_pow, in.fillin_tol, in.fillin_rank,
+                            precision, 0, &s_));
+  }
+  ~AIFMM() { if (s_) rcs_solver_destroy(s_); }
+  AIFMM(const AIFMM&) = delete;
+  AIFMM& operator=(const AIFMM&) = delete;
+  void factorize() { check(rcs_solver_factorize(s_)); }
+  Vec solve(Vec& rhs) {
+    const int n = (int)rhs.size();
+    phi_.assign(n, cdouble());
+    Vec xb(n);
+    check(rcs_solver_solve(s_, reinterpret_cast<const double*>(rhs.data()), 1, reinterpret_cast<double*>(phi_.data()),
+                           reinterpret_cast<double*>(xb.data())));
+    return xb;
+  }
+  void getPhi(Vec& phi) { phi = phi_; }
+  rcs_solver_t handle() const { return s_; }
+  kerneltype* K;
+
+ private:
+  rcs_solver_t s_ = nullptr;
+  Vec phi_;
+};
+
+}  // namespace rcs
+
+#endif

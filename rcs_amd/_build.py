@@ -14,10 +14,24 @@ SOURCES = ["entries.hip", "dense.hip", "rrqr.hip", "aca.hip", "tree.cpp", "capi.
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 instructions stay off (numerics)
+    # no packed FP32 instructions: some of their forms have given wrong results on this
+    # hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "driver_sphere.cpp")
+DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "driver_sphere")
+
+
+def build_driver():
+    """The C++ driver test over include/rcs_amd.hpp (host code only, g++)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC, "-L" + HERE, "-lrcs_amd",
+           "-Wl,-rpath,$ORIGIN/../../rcs_amd", "-o", DRIVER_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stderr)
+    return DRIVER_BIN
 
 
 def _compile(src):
@@ -39,8 +53,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER_BIN)
     return LIB
 
 

@@ -144,7 +144,7 @@ void lu_t(const double* A, int n, const double* B, int nrhs, double* X) {
   Ctx& cx = ctx();
   DMat<T> dA = upload<T>(cx, A, n, n, n);
   DMat<T> dB = upload<T>(cx, B, n, nrhs, n);
-  int* ip = (int*)cx.pool.alloc(sizeof(int) * std::max(n, 1));
+  int* ip = (int*)cx.pool.alloc(sizeof(int) * 2 * std::max(n, 1));  // pivots + permutation
   getrf(dA, ip, cx);
   getrs(dA, ip, dB, cx);
   download<T>(cx, dB, X);

@@ -208,14 +208,15 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
 // ---------------------------------------------------------------- LU factorization
 constexpr int LU_NB = 32;
 
-// Factor the panel A[j:n, j:j+nb] (one workgroup). Pivot = first row of maximal |a|.
+// Factor the panel A[j:n, j:j+nb] (one workgroup). Pivot = first row of maximal |a|
+// (largest value, lowest row index), as Eigen's PartialPivLU picks it.
 template <typename T>
 __global__ void __launch_bounds__(1024) lu_panel_kernel(T* A, int lda, int n, int j, int nb, int* ipiv) {
   using R = typename RealOf<T>::type;
-  __shared__ R sval[1024];
-  __shared__ int sidx[1024];
-  __shared__ int piv_s;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ R sval[16];
+  __shared__ int sidx[16];
+  __shared__ T urow[LU_NB];
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
   for (int jj = 0; jj < nb; ++jj) {
     const int c = j + jj;
     T* colc = A + (size_t)c * lda;
@@ -225,48 +226,39 @@ __global__ void __launch_bounds__(1024) lu_panel_kernel(T* A, int lda, int n, in
       R v = dabs(colc[r]);
       if (v > best) { best = v; bi = r; }
     }
-    sval[tid] = best;
-    sidx[tid] = bi;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        R v2 = sval[tid + s];
-        int i2 = sidx[tid + s];
-        if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) { sval[tid] = v2; sidx[tid] = i2; }
-      }
-      __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+      R ov = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
     }
-    if (tid == 0) {
-      int p = sidx[0];
-      if (p >= n) p = c;
-      piv_s = p;
-      ipiv[c] = p;
-    }
+    if (lane == 0) { sval[wave] = best; sidx[wave] = bi; }
     __syncthreads();
-    const int p = piv_s;
-    if (p != c) {
-      for (int q = tid; q < nb; q += nt) {
-        T* cq = A + (size_t)(j + q) * lda;
-        T tmp = cq[c];
-        cq[c] = cq[p];
-        cq[p] = tmp;
-      }
+    best = sval[0];
+    bi = sidx[0];
+    for (int w = 1; w < nw; ++w)
+      if (sval[w] > best || (sval[w] == best && sidx[w] < bi)) { best = sval[w]; bi = sidx[w]; }
+    const int p = (bi >= n) ? c : bi;
+    if (tid == 0) ipiv[c] = p;
+    // swap rows c and p inside the panel; keep row c (the new U row) in LDS
+    if (tid < nb) {
+      T* cq = A + (size_t)(j + tid) * lda;
+      T a = cq[c], b = cq[p];
+      if (p != c) { cq[c] = b; cq[p] = a; }
+      urow[tid] = (p != c) ? b : a;
     }
     __syncthreads();
-    const T piv = colc[c];
+    const T piv = urow[jj];
     const bool nz = !(piv.re == R(0) && piv.im == R(0));
-    if (nz) {
-      for (int r = c + 1 + tid; r < n; r += nt) colc[r] = cdiv(colc[r], piv);
-    }
-    __syncthreads();
-    // rank-1 update of the remaining panel columns
-    const int rows = n - c - 1, cols = nb - jj - 1;
-    for (int e = tid; e < rows * cols; e += nt) {
-      int r = c + 1 + e % rows, q = c + 1 + e / rows;
-      T* cq = A + (size_t)q * lda;
-      T f = cq[c];
+    // scale column c below the diagonal and update the remaining panel columns: thread
+    // owns rows r = c+1+tid+nt*k; the l factor is computed once per row
+    const int cols = nb - jj - 1;
+    for (int r = c + 1 + tid; r < n; r += nt) {
       T l = colc[r];
-      cq[r] = cq[r] - l * f;
+      if (nz) { l = cdiv(l, piv); colc[r] = l; }
+      for (int q = 0; q < cols; ++q) {
+        T* cq = A + (size_t)(c + 1 + q) * lda;
+        cq[r] = cq[r] - l * urow[jj + 1 + q];
+      }
     }
     __syncthreads();
   }
@@ -285,6 +277,33 @@ __global__ void laswp_kernel(T* A, int lda, int ncols, int j, int nb, const int*
       col[r] = col[p];
       col[p] = t;
     }
+  }
+}
+
+// perm = the row swaps ipiv[0..n) applied in order to the identity (one workgroup; the
+// swaps are sequential by definition, done in LDS when they fit).
+constexpr int PERM_LDS = 8192;
+__global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int* ipiv, int* perm, int n) {
+  __shared__ int pl[PERM_LDS];
+  __shared__ int il[PERM_LDS];
+  if (n <= PERM_LDS) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { pl[i] = i; il[i] = ipiv[i]; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int r = 0; r < n; ++r) {
+        int p = il[r];
+        if (p != r) { int t = pl[r]; pl[r] = pl[p]; pl[p] = t; }
+      }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = pl[i];
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int r = 0; r < n; ++r) {
+        int p = ipiv[r];
+        if (p != r) { int t = perm[r]; perm[r] = perm[p]; perm[p] = t; }
+      }
   }
 }
 
@@ -368,47 +387,47 @@ void getrf(DMat<T> A, int* ipiv, Ctx& cx) {
       gemm_grouped(g, cx);
     }
   }
+  hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3(1), dim3(256), 0, st, (const int*)ipiv, ipiv + n, n);
   HIPCHK(hipGetLastError());
 }
 
-// One workgroup per right-hand side column: row swaps, unit-lower then upper solve.
+// Few right-hand sides of a system of at most GETRS_LDS unknowns: one workgroup per
+// column, the column gathered through the permutation into LDS and solved there.
+constexpr int GETRS_LDS = 1024;
 template <typename T>
-__global__ void __launch_bounds__(256) getrs_vec_kernel(const T* LU, int ld, int n, const int* ipiv, T* B, int ldb) {
+__global__ void __launch_bounds__(256) getrs_vec_kernel(const T* LU, int ld, int n, const int* perm, T* B, int ldb) {
+  __shared__ T xl[GETRS_LDS];
   T* b = B + (size_t)blockIdx.x * ldb;
   const int tid = threadIdx.x;
-  if (tid == 0)
-    for (int r = 0; r < n; ++r) {
-      int p = ipiv[r];
-      if (p != r) { T t = b[r]; b[r] = b[p]; b[p] = t; }
-    }
+  for (int i = tid; i < n; i += blockDim.x) xl[i] = b[perm[i]];
   __syncthreads();
-  __shared__ T xs[64];
-  // forward, blocks of 64
+  // forward (unit lower), blocks of 64: one wavefront solves the diagonal block, then all
+  // threads update the rows below
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int nb = min(64, n - j0);
-    if (tid < 64) {  // one wavefront solves the diagonal block
-      T x = tid < nb ? b[j0 + tid] : T(0, 0);
+    if (tid < 64) {
+      T x = tid < nb ? xl[j0 + tid] : T(0, 0);
       for (int k = 0; k < nb; ++k) {
         T xk;
         xk.re = __shfl(x.re, k);
         xk.im = __shfl(x.im, k);
         if (tid > k && tid < nb) x = x - LU[(size_t)(j0 + k) * ld + j0 + tid] * xk;
       }
-      if (tid < nb) { b[j0 + tid] = x; xs[tid] = x; }
+      if (tid < nb) xl[j0 + tid] = x;
     }
     __syncthreads();
     for (int r = j0 + nb + tid; r < n; r += blockDim.x) {
-      T s = b[r];
-      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xs[k];
-      b[r] = s;
+      T s = xl[r];
+      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xl[j0 + k];
+      xl[r] = s;
     }
     __syncthreads();
   }
-  // backward
+  // backward (upper)
   for (int j1 = n; j1 > 0; j1 -= 64) {
     const int j0 = max(0, j1 - 64), nb = j1 - j0;
     if (tid < 64) {
-      T x = tid < nb ? b[j0 + tid] : T(0, 0);
+      T x = tid < nb ? xl[j0 + tid] : T(0, 0);
       for (int k = nb - 1; k >= 0; --k) {
         if (tid == k) x = cdiv(x, LU[(size_t)(j0 + k) * ld + j0 + k]);
         T xk;
@@ -416,27 +435,25 @@ __global__ void __launch_bounds__(256) getrs_vec_kernel(const T* LU, int ld, int
         xk.im = __shfl(x.im, k);
         if (tid < k) x = x - LU[(size_t)(j0 + k) * ld + j0 + tid] * xk;
       }
-      if (tid < nb) { b[j0 + tid] = x; xs[tid] = x; }
+      if (tid < nb) xl[j0 + tid] = x;
     }
     __syncthreads();
     for (int r = tid; r < j0; r += blockDim.x) {
-      T s = b[r];
-      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xs[k];
-      b[r] = s;
+      T s = xl[r];
+      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xl[j0 + k];
+      xl[r] = s;
     }
     __syncthreads();
   }
+  for (int i = tid; i < n; i += blockDim.x) b[i] = xl[i];
 }
 
 template <typename T>
-__global__ void laswp_all_kernel(T* B, int ldb, int ncols, int n, const int* ipiv) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  T* col = B + (size_t)c * ldb;
-  for (int r = 0; r < n; ++r) {
-    int p = ipiv[r];
-    if (p != r) { T t = col[r]; col[r] = col[p]; col[p] = t; }
-  }
+__global__ void permute_rows_kernel(T* d, int ldd, const T* s, int lds, int r, int c, const int* perm) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)r * c) return;
+  int i = e % r, j = e / r;
+  d[(size_t)i + (size_t)j * ldd] = s[(size_t)perm[i] + (size_t)j * lds];
 }
 
 template <typename T>
@@ -445,15 +462,24 @@ void getrs(DMat<T> LU, const int* ipiv, DMat<T> B, Ctx& cx) {
   const int n = LU.r;
   if (n == 0 || B.c == 0) return;
   if (B.r != n) throw std::runtime_error("getrs: size mismatch");
+  const int* perm = ipiv + n;
   // a few right-hand sides of a small system: one workgroup per column is fastest; large
   // systems take the blocked path below, whose updates spread over the whole chip
-  if (B.c <= 8 && n <= 1024) {
-    hipLaunchKernelGGL(getrs_vec_kernel<T>, dim3(B.c), dim3(256), 0, st, (const T*)LU.p, LU.ld, n, ipiv, B.p, B.ld);
+  if (B.c <= 8 && n <= GETRS_LDS) {
+    hipLaunchKernelGGL(getrs_vec_kernel<T>, dim3(B.c), dim3(256), 0, st, (const T*)LU.p, LU.ld, n, perm, B.p, B.ld);
     HIPCHK(hipGetLastError());
     return;
   }
   const int nc = B.c;
-  hipLaunchKernelGGL(laswp_all_kernel<T>, dim3((nc + 127) / 128), dim3(128), 0, st, B.p, B.ld, nc, n, ipiv);
+  {  // B <- P B through a gathered copy
+    T* tmp = (T*)cx.pool.alloc(sizeof(T) * (size_t)n * nc);
+    DMat<T> Tm{tmp, n, nc, n};
+    const size_t cnt = (size_t)n * nc;
+    hipLaunchKernelGGL(permute_rows_kernel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, tmp, n,
+                       (const T*)B.p, B.ld, n, nc, perm);
+    copy2d(B, Tm, st);
+    cx.pool.free(tmp);
+  }
   for (int j = 0; j < n; j += LU_NB) {
     const int nb = std::min(LU_NB, n - j);
     hipLaunchKernelGGL(trsm_lower_unit_kernel<T>, dim3((nc + TRSM_COLS - 1) / TRSM_COLS), dim3(256), 0, st,

@@ -46,8 +46,10 @@ struct GemmOp {
 template <typename T>
 void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role = 0, int prof_tag = -1);
 
-// In-place LU with partial pivoting (first maximal |a| in the column), ipiv 0-based
-// absolute row swapped with each row (LAPACK style). ipiv is a device array of n ints.
+// In-place LU with partial pivoting (first maximal |a| in the column). ipiv is a device
+// array of 2n ints: ipiv[0, n) = 0-based absolute row swapped with each row (LAPACK
+// style), ipiv[n, 2n) = the composed permutation (row i of PA is row perm[i] of A),
+// which lets solves gather the right-hand side in parallel.
 template <typename T>
 void getrf(DMat<T> A, int* ipiv, Ctx& cx);
 // Solve (LU) X = B in place of B.

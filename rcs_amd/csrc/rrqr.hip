@@ -84,6 +84,16 @@ __device__ inline void rq_setup_phase2(RRQRState& s) {
   s.phase = 2;
 }
 
+// Acceptance test of step j alone (every thread evaluates it; no state copy).
+__device__ inline int rq_accept(const RRQRState& s, int j, double pivot) {
+  if (s.phase == 1) {
+    const double max1 = (j == 0) ? pivot : s.max1;
+    return (pivot / max1 >= s.tol1) ? 1 : 0;
+  }
+  const double mp = (j == s.first) ? fmax(s.inputPivot, pivot) : s.maxPivot;
+  return (pivot / mp >= s.tol2 && !(j - s.first >= s.fillInRank)) ? 1 : 0;
+}
+
 // Step decision (src/rrqr.cpp:108-121 and 155-170): returns 1 if column j is accepted;
 // u receives the next state.
 __device__ inline int rq_decide(const RRQRState& s, int j, double pivot, RRQRState& u) {
@@ -141,6 +151,7 @@ __global__ void __launch_bounds__(256) rrqr_norms_kernel(RRQRJobs<T> jobs, RRQRI
 }
 
 constexpr int RQ_XS = 2048;  // pivot-column rows staged in LDS (longer columns read HBM)
+constexpr int RQ_RC = 16;    // column rows per lane kept in registers (columns of <= 1024 rows)
 
 // One pivot step of every job. Launch it reads state slot it&1 and writes (it+1)&1.
 template <typename T>
@@ -183,12 +194,34 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
   const int mlo = sh_mlo, mhi = sh_mhi;  // norms of [mlo, mhi) read as zero in this step
   const int end = (s.phase == 1) ? s.partition : n;
   const double* nsi = jb.ns[buf];
+  const T* Ain = jb.A[buf];
+  // this wavefront's column, rows j + lane + 64 t, loaded into registers now so that the
+  // loads overlap the pivot search (positions j and p are reloaded once p is known)
+  const int c = j + wgl * 4 + wave;
+  const bool cached = m - j <= 64 * RQ_RC;
+  T ar[RQ_RC];
+  if (cached) {
+#pragma unroll
+    for (int t = 0; t < RQ_RC; ++t) {
+      const int i = j + lane + 64 * t;
+      ar[t] = (c < n && i < m) ? Ain[(size_t)c * m + i] : T(0, 0);
+    }
+  }
   // first maximal downdated norm over [j, end): (largest value, lowest index)
   double best = -1.0;
   int bi = end;
-  for (int c = j + tid; c < end; c += 256) {
-    double v = (c >= mlo && c < mhi) ? 0.0 : nsi[c];
-    if (v > best) { best = v; bi = c; }
+  for (int c0 = j; c0 < end; c0 += 1024) {  // four loads per thread in flight together
+    double nv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = c0 + tid + 256 * t;
+      nv[t] = c < end ? ((c >= mlo && c < mhi) ? 0.0 : nsi[c]) : -2.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = c0 + tid + 256 * t;
+      if (c < end && nv[t] > best) { best = nv[t]; bi = c; }
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     double ob = __shfl_xor(best, off);
@@ -204,10 +237,10 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
     if (rv[w] > best || (rv[w] == best && ri[w] < bi)) { best = rv[w]; bi = ri[w]; }
   const int p = (bi >= end) ? j : bi;
   // pivot column p, rows j..m-1, staged in LDS, and its norm
-  const T* Ain = jb.A[buf];
   const T* x = Ain + (size_t)p * m;
   const bool inl = m <= RQ_XS;
   double sn = 0;
+#pragma unroll 4
   for (int i = j + tid; i < m; i += 256) {
     T xv = x[i];
     if (inl) xs[i] = xv;
@@ -236,19 +269,55 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
   const T vj = vel(j);
   const double dr = (double)xj.re - ((double)vj.re * hr - (double)vj.im * hi);
   const double di = (double)xj.im - ((double)vj.re * hi + (double)vj.im * hr);
-  RRQRState u;
-  const int acc = rq_decide(s, j, sqrt(dr * dr + di * di), u);
-  if (!acc) {
-    if (wgl == 0 && tid == 0) *out = u;
-    return;
+  const double pivot = sqrt(dr * dr + di * di);
+  const int acc = rq_accept(s, j, pivot);
+  if (wgl == 0 && tid == 0) {  // the next state, written once per job
+    RRQRState u;
+    rq_decide(s, j, pivot, u);
+    *out = u;
   }
+  if (!acc) return;
   if (wgl == 0) {  // store the accepted reflector
     T* v = jb.V + (size_t)s.nrefl * m;
     for (int i = tid; i < m; i += 256) v[i] = (i < j) ? T(0, 0) : vel(i);
   }
   // apply to logical columns j..n-1, one wavefront per column, and downdate the norms
-  const int c = j + wgl * 4 + wave;
-  if (c < n) {
+  if (c < n && cached) {
+    const int src = (c == j) ? p : (c == p ? j : c);
+    if (src != c) {
+#pragma unroll
+      for (int t = 0; t < RQ_RC; ++t) {
+        const int i = j + lane + 64 * t;
+        ar[t] = i < m ? Ain[(size_t)src * m + i] : T(0, 0);
+      }
+    }
+    T vr[RQ_RC];
+    double wr = 0, wi = 0;
+#pragma unroll
+    for (int t = 0; t < RQ_RC; ++t) {
+      const int i = j + lane + 64 * t;
+      vr[t] = i < m ? vel(i) : T(0, 0);
+      wr += (double)vr[t].re * ar[t].re + (double)vr[t].im * ar[t].im;
+      wi += (double)vr[t].re * ar[t].im - (double)vr[t].im * ar[t].re;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      wr += __shfl_xor(wr, off);
+      wi += __shfl_xor(wi, off);
+    }
+    const T w((R)wr, (R)wi);
+    T* ao = jb.A[buf ^ 1] + (size_t)c * m;
+#pragma unroll
+    for (int t = 0; t < RQ_RC; ++t) {
+      const int i = j + lane + 64 * t;
+      if (i < m) ao[i] = (c == j && i > j) ? T(0, 0) : ar[t] - vr[t] * w;
+    }
+    if (c != j && lane == 0) {
+      const T d = ar[0] - vr[0] * w;
+      const double dd = sqrt((double)d.re * d.re + (double)d.im * d.im);
+      const double nsrc = (src >= mlo && src < mhi) ? 0.0 : nsi[src];
+      jb.ns[buf ^ 1][c] = nsrc - dd * dd;
+    }
+  } else if (c < n) {
     const int src = (c == j) ? p : (c == p ? j : c);
     const T* a = Ain + (size_t)src * m;
     T* ao = jb.A[buf ^ 1] + (size_t)c * m;
@@ -277,7 +346,6 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
       }
     }
   }
-  if (wgl == 0 && tid == 0) *out = u;
 }
 
 // Q(:, c) = H_0 H_1 ... H_{nrefl-1} e_c, one workgroup per output column of every job.

@@ -157,7 +157,7 @@ class Solver : public SolverBase {
   void factor(DMat<T> A, DMat<T>& lu, int*& ip) {
     put(lu, clone(A));
     if (ip) cx.pool.free(ip);
-    ip = (int*)cx.pool.alloc(sizeof(int) * std::max(A.r, 1));
+    ip = (int*)cx.pool.alloc(sizeof(int) * 2 * std::max(A.r, 1));  // pivots + permutation
     double t0 = wall();
     int h = cx.prof.begin(cx.st);
     getrf(lu, ip, cx);

@@ -221,3 +221,41 @@ def test_aifmm_multi_rhs_and_c64(rng):
     s32 = R.AIFMM(g, M.bounding_box(g.locations()), 64, 8, 8, 0, R.C64)
     s32.factorize()
     assert _rel(s32.solve(b[:, 0]), xo) < 2e-3
+
+
+def test_cpp_driver_matches_oracle(fixture78, tmp_path):
+    """The reference driver's flow (test/sphere.cpp) through the C++ drop-in interface
+    include/rcs_amd.hpp (tests/cpp/driver_sphere.cpp), checked against the oracle."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "cpp", "driver_sphere")
+    assert os.path.exists(exe), "tests/cpp/driver_sphere is built by __graft_entry__.build()"
+    golden = os.path.join(root, "tests", "golden")
+    out = str(tmp_path / "d")
+    r = subprocess.run([exe, os.path.join(golden, "nodeList78.txt"), os.path.join(golden, "patchList78.txt"), "1e9",
+                        "6", "64", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    load = lambda f: (lambda a: a[:, 0] + 1j * a[:, 1])(np.loadtxt(f, ndmin=2))
+    phi, xbox = load(out + "_phi.txt"), load(out + "_box.txt")
+    nodes, tris = fixture78
+    o = O.Mesh(nodes, tris)
+    loc = o.rwg_midpoints()
+    b = o.rhs(1e9)
+    so = O.AIFMM(o, O.EFIE, 1e9, loc, M.bounding_box(loc), 64, 6, 6)
+    so.factorize()
+    xo, xob = so.solve(b, box_order=True)
+    assert _rel(phi, xo) < 1e-10 and _rel(xbox, xob) < 1e-10
+    th_rcs = np.loadtxt(out + "_rcs.txt")
+    assert _rel(th_rcs[:, 1], o.rcs(1e9, xo, th_rcs[:, 0], np.zeros(181))) < 1e-8
+    n = len(xo)
+    rows, cols = np.arange(n // 3), np.arange(2 * n // 3, n)
+    lines = [list(map(int, ln.split())) for ln in open(out + "_aca.txt") if ln.strip()]
+    ro, rbo, cbo = o.aca(O.EFIE, 1e9, rows, cols, 1e-6)
+    assert lines[0] == [ro, rows.size, ro, ro, cols.size]
+    bases = np.array(lines[1:], dtype=np.int64).reshape(-1, 2)
+    assert np.array_equal(bases[:, 0], rbo) and np.array_equal(bases[:, 1], cbo)
+    B = o.entries(O.EFIE, 1e9, rows, cols)
+    _, _, _, rq, mpo = O.rrqr(B, 6, 6, 3, 0.0, 1000)
+    rk, mp = np.loadtxt(out + "_rrqr.txt")
+    assert int(rk) == rq and abs(mp - mpo) < 1e-10 * mpo
