@@ -88,6 +88,24 @@ int rcs_solver_factorize(rcs_solver_t s);
  * (N x nrhs complex). x gets getPhi's input order; x_box (may be NULL) gets solve()'s
  * box order (src/AIFMM.cpp:103-110). */
 int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, double* x_box);
+/* FP64 compressed operator for residuals: the AIFMM<K> assembly (src/AIFMM.cpp:6-93, tree
+ * set-up, nested ACA skeletons, M2L, leaf P2P) at its own tolerance, never factorized, and
+ * applied as y = Z x through the near field plus the nested far field (upward P2M, M2L,
+ * downward L2P). No counterpart in the reference, whose driver measures the error of a
+ * solve against the dense matrix (test/sphere.cpp:331-338); it is the FP64 residual of the
+ * mixed-precision path. x, y: N x nrhs complex, input order. */
+typedef struct rcs_operator_s* rcs_operator_t;
+int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, rcs_operator_t* out);
+void rcs_operator_destroy(rcs_operator_t o);
+int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y);
+/* Restarted GMRES(restart) on Z x = rhs with the FP64 operator o, right-preconditioned by
+ * the factorized solver s (any precision), CGS2 orthogonalisation, until
+ * ||rhs - Z x|| <= tol ||rhs|| or maxit iterations. x: N x nrhs complex (input order).
+ * iters / relres (may be NULL): largest iteration count and final relative residual.
+ * No counterpart in the reference (see rcs_operator_create). */
+int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int nrhs, double* x, double tol, int maxit,
+                      int restart, int* iters, double* relres);
+
 /* Statistics (the driver's prints, test/sphere.cpp:217-262):
  * info[0..4] = nLevels, startLevel, n_rrqr, n_compress_p2p, n_compress_p2l;
  * vals[0..10] = t_assemble, t_factor, t_solve, gemm_flops, getrf_flops, getrs_flops, qr_flops,

@@ -55,6 +55,10 @@ def lib():
         L.rcs_solver_ranks.argtypes = [vp, C.c_int, ip, C.c_int]
         L.rcs_profile_enable.argtypes = [C.c_int]
         L.rcs_profile_get.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_longlong)]
+        L.rcs_operator_create.argtypes = [vp, dp, C.c_int, C.c_int, C.POINTER(vp)]
+        L.rcs_operator_destroy.argtypes = [vp]
+        L.rcs_operator_apply.argtypes = [vp, dp, C.c_int, dp]
+        L.rcs_solver_refine.argtypes = [vp, vp, dp, C.c_int, dp, C.c_double, C.c_int, C.c_int, ip, dp]
         _LIB = L
     return _LIB
 
@@ -64,6 +68,7 @@ EXPORTED = [
     "rcs_mesh_rwg", "rcs_entries", "rcs_rhs", "rcs_rcs", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
     "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
     "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
+    "rcs_operator_create", "rcs_operator_destroy", "rcs_operator_apply", "rcs_solver_refine",
 ]
 
 PROFILE_TAGS = ["schur", "getrf", "getrs", "entries", "aca", "rrqr", "update"]
@@ -267,6 +272,50 @@ class AIFMM:
         if n < 0:
             _check(n)
         return out[: 3 * n].reshape(n, 3)
+
+    def refine(self, op, rhs, tol=1e-6, maxit=50, restart=30):
+        """GMRES on Z x = rhs with the FP64 operator op, preconditioned by this
+        factorization. Returns (x, iterations, final relative residual)."""
+        rhs = np.asarray(rhs, dtype=np.complex128)
+        single = rhs.ndim == 1
+        R = rhs.reshape(self.N, -1)
+        nrhs = R.shape[1]
+        Rf = R.reshape(-1, order="F").copy()
+        X = np.zeros_like(Rf)
+        it = np.zeros(1, dtype=np.int32)
+        rr = np.zeros(1)
+        _check(lib().rcs_solver_refine(self.h, op.h, _cplx_ptr(Rf), nrhs, _cplx_ptr(X), tol, maxit, restart,
+                                       _ip(it), _dp(rr)))
+        X = X.reshape((self.N, nrhs), order="F")
+        return (X[:, 0] if single else X), int(it[0]), float(rr[0])
+
+
+class Operator:
+    """FP64 compressed operator (assembly only) for residuals: apply(x) = Z x."""
+
+    def __init__(self, mesh_: Mesh, bbox, leaf_size=64, tol_pow=10):
+        bbox = np.ascontiguousarray(bbox, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().rcs_operator_create(mesh_.h, _dp(bbox), leaf_size, tol_pow, C.byref(h)))
+        self.h = h
+        self.mesh = mesh_
+        self.N = mesh_.N
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rcs_operator_destroy(self.h)
+            self.h = None
+
+    def apply(self, x):
+        x = np.asarray(x, dtype=np.complex128)
+        single = x.ndim == 1
+        Xr = x.reshape(self.N, -1)
+        nrhs = Xr.shape[1]
+        Xf = Xr.reshape(-1, order="F").copy()
+        Y = np.zeros_like(Xf)
+        _check(lib().rcs_operator_apply(self.h, _cplx_ptr(Xf), nrhs, _cplx_ptr(Y)))
+        Y = Y.reshape((self.N, nrhs), order="F")
+        return Y[:, 0] if single else Y
 
 
 def solver_block(solver, level, box, which, key=0, cap=4_000_000):

@@ -10,6 +10,7 @@
 #include "dense.hpp"
 #include "entries.hpp"
 #include "geometry.hpp"
+#include "refine.hpp"
 #include "tree.hpp"
 
 using namespace rcs;
@@ -19,6 +20,10 @@ struct rcs_mesh_s {
 };
 struct rcs_solver_s {
   std::unique_ptr<SolverBase> s;
+  std::shared_ptr<DeviceGeometry> g;
+};
+struct rcs_operator_s {
+  std::unique_ptr<SolverBase> s;  // assembled, never factorized
   std::shared_ptr<DeviceGeometry> g;
 };
 
@@ -369,6 +374,55 @@ int rcs_solver_factorize(rcs_solver_t s) {
 int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, double* x_box) {
   TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
   s->s->solve(rhs, nrhs, x, x_box);
+  return 0;
+  CATCH
+}
+int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, rcs_operator_t* out) {
+  TRY if (leaf_size <= 0) throw std::runtime_error("leaf size must be positive");
+  SolverParams p;
+  for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];
+  p.leafSize = leaf_size;
+  p.tol_pow = tol_pow;
+  p.fillin_tol = tol_pow;
+  std::unique_ptr<rcs_operator_s> o(new rcs_operator_s);
+  o->g = m->g;
+  o->s = make_solver(m->g, p, RCS_PREC_C128, ctx());
+  o->s->assemble();
+  *out = o.release();
+  return 0;
+  CATCH
+}
+void rcs_operator_destroy(rcs_operator_t o) { delete o; }
+int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
+  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  Ctx& cx = ctx();
+  const size_t bytes = sizeof(cpx<double>) * (size_t)o->s->size() * nrhs;
+  void* dx = cx.pool.alloc(bytes);
+  void* dy = cx.pool.alloc(bytes);
+  cx.upload(dx, x, bytes);
+  o->s->apply_device(dx, nrhs, dy);
+  HIPCHK(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, cx.st));
+  cx.sync();
+  cx.pool.free(dx);
+  cx.pool.free(dy);
+  return 0;
+  CATCH
+}
+int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int nrhs, double* x, double tol, int maxit,
+                      int restart, int* iters, double* relres) {
+  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  Ctx& cx = ctx();
+  const size_t bytes = sizeof(cpx<double>) * (size_t)s->s->size() * nrhs;
+  cpx<double>* db = (cpx<double>*)cx.pool.alloc(bytes);
+  cpx<double>* dx = (cpx<double>*)cx.pool.alloc(bytes);
+  cx.upload(db, rhs, bytes);
+  RefineResult r = gmres_refine(*o->s, *s->s, db, nrhs, dx, tol, maxit, restart, cx);
+  HIPCHK(hipMemcpyAsync(x, dx, bytes, hipMemcpyDeviceToHost, cx.st));
+  cx.sync();
+  cx.pool.free(db);
+  cx.pool.free(dx);
+  if (iters) *iters = r.iters;
+  if (relres) *relres = r.relres;
   return 0;
   CATCH
 }

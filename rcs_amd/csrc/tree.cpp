@@ -1165,7 +1165,112 @@ class Solver : public SolverBase {
     trace("factorization done");
     stats_.t_factor = wall() - t0;
     stats_.peak_bytes = cx.pool.peak();
+    factorized_ = true;
   }
+
+  // ------------------------------------------------------------ operator apply
+  // y = Z x with the assembled (not yet factorized) compressed operator: the leaf near
+  // field P2P plus the nested far field L2P M2L P2M (upward pass through P2M, M2L at every
+  // level, downward pass through L2P), every level one grouped launch. x and y are device
+  // complex128, N x nrhs (ld N), in input order.
+  void apply_device(const void* x_dev, int nrhs, void* y_dev) override {
+    if (factorized_) throw std::runtime_error("operator apply after factorize (the blocks were eliminated)");
+    const int jl = nLevels;
+    // box order: the concatenated leaf charge lists
+    std::vector<int> order, poff(nBoxes[jl] + 1, 0);
+    for (int k = 0; k < nBoxes[jl]; ++k) {
+      const auto& ch = tree[jl][k].charge;
+      order.insert(order.end(), ch.begin(), ch.end());
+      poff[k + 1] = poff[k] + (int)ch.size();
+    }
+    DMat<T> X = newmat(N, nrhs), Xb = newmat(N, nrhs), Yb = newmat(N, nrhs);
+    to_T(X, DMat<cpx<double>>{(cpx<double>*)x_dev, N, nrhs, N});
+    int* d_order = (int*)cx.pool.alloc(sizeof(int) * std::max(N, 1));  // lives across many launches
+    cx.upload(d_order, order.data(), sizeof(int) * order.size());
+    gather_rows(Xb, X, d_order, cx.st);
+    // per level contiguous multipole / local buffers, box after box
+    std::vector<std::vector<int>> moff(nLevels + 1), loff(nLevels + 1);
+    std::vector<DMat<T>> mult(nLevels + 1), loc(nLevels + 1);
+    for (int j = startLevel; j <= nLevels; ++j) {
+      moff[j].assign(nBoxes[j] + 1, 0);
+      loff[j].assign(nBoxes[j] + 1, 0);
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        moff[j][k + 1] = moff[j][k] + tree[j][k].P2M.r;
+        loff[j][k + 1] = loff[j][k] + tree[j][k].L2P.c;
+      }
+      mult[j] = newmat(moff[j][nBoxes[j]], nrhs);
+      loc[j] = zeros(loff[j][nBoxes[j]], nrhs);
+    }
+    auto slice = [&](DMat<T>& m, int r0, int nr) { return m.rows(r0, nr); };
+    // upward: leaf P2M on the particles, then P2M on the children's multipoles
+    for (int j = nLevels; j >= startLevel; --j) {
+      Group g;
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        const Box<T>& b = tree[j][k];
+        if (b.P2M.size() == 0) continue;
+        DMat<T> src = (j == nLevels) ? slice(Xb, poff[k], poff[k + 1] - poff[k])
+                                     : slice(mult[j + 1], moff[j + 1][8 * k], moff[j + 1][8 * k + 8] - moff[j + 1][8 * k]);
+        if (src.r != b.P2M.c) throw std::runtime_error("apply: P2M does not match its sources");
+        add(g, slice(mult[j], moff[j][k], b.P2M.r), b.P2M, OP_N, src, OP_N, T(1, 0), T(0, 0));
+      }
+      run(g);
+    }
+    // M2L at every level: locals of k from the multipoles of its interaction list
+    for (int j = startLevel; j <= nLevels; ++j) {
+      Group g;
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        Box<T>& b = tree[j][k];
+        const int nl = b.L2P.c;
+        if (nl == 0) continue;
+        for (int l : b.IL) {
+          auto it = b.M2L.find(l);
+          if (it == b.M2L.end() || it->second.size() == 0) continue;
+          add(g, slice(loc[j], loff[j][k], nl), it->second, OP_N, slice(mult[j], moff[j][l], tree[j][l].P2M.r), OP_N,
+              T(1, 0), T(1, 0), 1);
+        }
+      }
+      run(g);
+    }
+    // downward: L2P of a parent adds to its children's locals
+    for (int j = startLevel; j < nLevels; ++j) {
+      Group g;
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        const Box<T>& b = tree[j][k];
+        if (b.L2P.size() == 0) continue;
+        DMat<T> dst = slice(loc[j + 1], loff[j + 1][8 * k], loff[j + 1][8 * k + 8] - loff[j + 1][8 * k]);
+        if (dst.r != b.L2P.r) throw std::runtime_error("apply: L2P does not match the children's locals");
+        add(g, dst, b.L2P, OP_N, slice(loc[j], loff[j][k], b.L2P.c), OP_N, T(1, 0), T(1, 0));
+      }
+      run(g);
+    }
+    // leaf: particles from locals plus the near field
+    {
+      Group g;
+      zero2d(Yb, cx.st);
+      for (int k = 0; k < nBoxes[jl]; ++k) {
+        Box<T>& b = tree[jl][k];
+        const int n = poff[k + 1] - poff[k];
+        if (n == 0) continue;
+        DMat<T> yk = slice(Yb, poff[k], n);
+        if (b.L2P.size() > 0)
+          add(g, yk, b.L2P, OP_N, slice(loc[jl], loff[jl][k], b.L2P.c), OP_N, T(1, 0), T(1, 0), 1);
+        for (int nn : b.NB) {
+          auto it = b.P2P.find(nn);
+          if (it == b.P2P.end() || it->second.size() == 0) continue;
+          add(g, yk, it->second, OP_N, slice(Xb, poff[nn], poff[nn + 1] - poff[nn]), OP_N, T(1, 0), T(1, 0), 1);
+        }
+      }
+      run(g);
+    }
+    DMat<T> Y = newmat(N, nrhs);
+    scatter_rows(Y, Yb, d_order, cx.st);
+    from_T(DMat<cpx<double>>{(cpx<double>*)y_dev, N, nrhs, N}, Y);
+    for (int j = startLevel; j <= nLevels; ++j) { drop(mult[j]); drop(loc[j]); }
+    drop(X); drop(Xb); drop(Yb); drop(Y);
+    cx.pool.free(d_order);
+  }
+  int size() const override { return N; }
+  Ctx& context() override { return cx; }
   // T:4772-4810: the base-level system is factored once here; the reference builds
   // and factors it inside every back substitution.
   void factor_base_level() {
@@ -1211,17 +1316,28 @@ class Solver : public SolverBase {
     return v;
   }
   void solve(const double* rhs, int nrhs, double* x, double* x_box) override {
+    DMat<cpx<double>> B64{(cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
+    DMat<cpx<double>> X64{(cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
+    DMat<cpx<double>> XB64{x_box ? (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs) : nullptr,
+                           N, nrhs, N};
+    cx.upload(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs);
+    solve_device(B64.p, nrhs, X64.p, XB64.p);
+    HIPCHK(hipMemcpyAsync(x, X64.p, sizeof(cpx<double>) * X64.size(), hipMemcpyDeviceToHost, cx.st));
+    if (x_box)
+      HIPCHK(hipMemcpyAsync(x_box, XB64.p, sizeof(cpx<double>) * XB64.size(), hipMemcpyDeviceToHost, cx.st));
+    cx.sync();
+    cx.pool.free(B64.p);
+    cx.pool.free(X64.p);
+    if (XB64.p) cx.pool.free(XB64.p);
+  }
+  // rhs / x / x_box: device complex128, N x nrhs (ld N); x in input order, x_box (may be
+  // null) in box order. Stream ordered; no host synchronisation.
+  void solve_device(const void* rhs_dev, int nrhs, void* x_dev, void* xbox_dev) override {
+    if (!factorized_) throw std::runtime_error("solve before factorize");
     double t0 = wall();
     // assign_Leaf_rhs (T:670-678)
     DMat<T> B = newmat(N, nrhs);
-    {
-      DMat<cpx<double>> B64;
-      B64.r = N; B64.c = nrhs; B64.ld = N;
-      B64.p = (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs);
-      cx.upload(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs);
-      to_T(B, B64);
-      cx.pool.free(B64.p);
-    }
+    to_T(B, DMat<cpx<double>>{(cpx<double>*)rhs_dev, N, nrhs, N});
     for (int k = 0; k < nBoxes[nLevels]; ++k) {
       Box<T>& b = tree[nLevels][k];
       put(b.particle_rhs, newmat((int)b.charge.size(), nrhs));
@@ -1362,15 +1478,16 @@ class Solver : public SolverBase {
     copy_batched(cops, cx);
     DMat<T> Xo = zeros(N, nrhs);
     scatter_rows(Xo, Xb, upload_ints(order), cx.st);
+    from_T(DMat<cpx<double>>{(cpx<double>*)x_dev, N, nrhs, N}, Xo);
+    if (xbox_dev) from_T(DMat<cpx<double>>{(cpx<double>*)xbox_dev, N, nrhs, N}, Xb);
     cx.sync();
     stats_.t_solve = wall() - ts;
-    download(Xo, x);
-    if (x_box) download(Xb, x_box);
     free_tmp_ints();
     drop(Xb); drop(Xo); drop(B);
     (void)t0;
   }
   void to_T(DMat<T> dst, DMat<cpx<double>> src);
+  void from_T(DMat<cpx<double>> dst, DMat<T> src);
   void download(DMat<T> src, double* host);
 
   SolverStats stats() const override {
@@ -1442,6 +1559,7 @@ class Solver : public SolverBase {
   SolverParams P;
   Ctx& cx;
   SolverStats stats_;
+  bool factorized_ = false;
   int N = 0, nLevels = 0, startLevel = 0, TOL_POW = 6, fill_in_TOL_POW = 6;
   double L = 0, eta = 0, fillInTolerance = 0;
   int rankVec[32];
@@ -1475,6 +1593,10 @@ template <>
 void Solver<cpx<double>>::to_T(DMat<cpx<double>> dst, DMat<cpx<double>> src) { copy2d(dst, src, cx.st); }
 template <>
 void Solver<cpx<float>>::to_T(DMat<cpx<float>> dst, DMat<cpx<double>> src) { cast_c128_to_c64(dst, src, cx.st); }
+template <>
+void Solver<cpx<double>>::from_T(DMat<cpx<double>> dst, DMat<cpx<double>> src) { copy2d(dst, src, cx.st); }
+template <>
+void Solver<cpx<float>>::from_T(DMat<cpx<double>> dst, DMat<cpx<float>> src) { cast_c64_to_c128(dst, src, cx.st); }
 template <>
 void Solver<cpx<double>>::download(DMat<cpx<double>> src, double* host) {
   HIPCHK(hipMemcpyAsync(host, src.p, sizeof(cpx<double>) * src.size(), hipMemcpyDeviceToHost, cx.st));

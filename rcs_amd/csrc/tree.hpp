@@ -53,6 +53,12 @@ class SolverBase {
   virtual void factorize() = 0;
   // rhs and x are interleaved complex128 arrays of N x nrhs (column-major), x in input order
   virtual void solve(const double* rhs, int nrhs, double* x, double* x_box) = 0;
+  // The same on device complex128 arrays (ld N), stream ordered; x_box may be null.
+  virtual void solve_device(const void* rhs, int nrhs, void* x, void* x_box) = 0;
+  // y = Z x with the assembled compressed operator (before factorize), device arrays.
+  virtual void apply_device(const void* x, int nrhs, void* y) = 0;
+  virtual int size() const = 0;
+  virtual Ctx& context() = 0;
   virtual SolverStats stats() const = 0;
   virtual void ranks(int level, std::vector<int>& out) = 0;
   // Copy one stored block to the host as complex128 (which: 0 L2P, 1 P2M, 2 M2L[key],

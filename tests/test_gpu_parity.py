@@ -259,3 +259,43 @@ def test_cpp_driver_matches_oracle(fixture78, tmp_path):
     _, _, _, rq, mpo = O.rrqr(B, 6, 6, 3, 0.0, 1000)
     rk, mp = np.loadtxt(out + "_rrqr.txt")
     assert int(rk) == rq and abs(mp - mpo) < 1e-10 * mpo
+
+
+@pytest.mark.parametrize("kind,tp,bar", [(R.EFIE, 10, 1e-8), (R.EFIE, 6, 1e-4), (R.LAPLACE, 10, 2e-6)])
+def test_operator_apply_matches_dense(rng, kind, tp, bar):
+    """The FP64 compressed operator (near P2P + nested L2P M2L P2M) against the dense
+    matrix-vector product of the oracle; the error follows the ACA tolerance. (For the
+    Laplace collocation the incoming and outgoing skeletons differ more in size, and the
+    reference's rank equalisation (T:682-689) truncates the larger one, so TOL 10 gives
+    about 1e-6; the oracle's own Laplace solve at TOL 10 is 3e-5 from dense.)"""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    g = R.Mesh(nodes, tris, kind, 1e9)
+    o = O.Mesh(nodes, tris)
+    op = R.Operator(g, M.bounding_box(g.locations()), 64, tp)
+    X = rng.normal(size=(g.N, 3)) + 1j * rng.normal(size=(g.N, 3))
+    Y = op.apply(X)
+    ok = O.EFIE if kind == R.EFIE else O.LAPLACE
+    for c in range(3):
+        assert _rel(Y[:, c], o.matvec(ok, 1e9, X[:, c])) < bar
+
+
+@pytest.mark.parametrize("prec,tp_factor", [(R.C64, 6), (R.C64, 4), (R.C128, 4)])
+def test_refinement_reaches_dense_solution(prec, tp_factor):
+    """Mixed precision: a loose (complex64 or complex128) factorization preconditions GMRES
+    with the FP64 operator at TOL 12; the refined solution matches the dense LU solution of
+    the oracle far below the factorization's own accuracy."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+    o = O.Mesh(nodes, tris)
+    bb = M.bounding_box(g.locations())
+    b = g.rhs()
+    xd = o.dense_solve(O.EFIE, 1e9, b)
+    s = R.AIFMM(g, bb, 64, tp_factor, tp_factor, 0, prec)
+    s.factorize()
+    x0 = s.solve(b)
+    op = R.Operator(g, bb, 64, 12)
+    x, it, rr = s.refine(op, b, tol=1e-10, maxit=60, restart=30)
+    print(f"prec={prec} TOL={tp_factor}: unrefined {_rel(x0, xd):.2e}, refined {_rel(x, xd):.2e} "
+          f"after {it} GMRES iterations (relres {rr:.1e})")
+    assert rr <= 1e-10
+    assert _rel(x, xd) < 1e-8 and _rel(x, xd) < 1e-2 * _rel(x0, xd)
