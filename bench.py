@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--config", default="efie-sphere-30720", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the accuracy / mixed-precision comparison")
+    ap.add_argument("--mixed-tol", type=int, default=4, help="TOL_POW of the complex64 factorization")
+    ap.add_argument("--plain-tol-accurate", type=int, default=9, help="TOL_POW of the accurate plain FP64 run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -204,9 +207,57 @@ def main():
                                 "work": v["flops"] / args.steps} for k, v in prof.items()},
         "rcs_theta0_dbsm": float(10 * np.log10(rcs[0])),
     }
+    if not args.no_mixed:
+        del s
+        out.update(accuracy_and_mixed(R, M, g, bb, b, x, freq, args, log))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
     print(json.dumps(out))
+
+
+def accuracy_and_mixed(R, M, g, bb, b, x_plain, freq, args, log):
+    """Accuracy of the plain path and the mixed-precision form on the same workload.
+
+    The yardstick is the relative residual ||b - Z x|| / ||b|| with an FP64 compressed
+    operator at TOL_POW 10 (assembled once, timed separately). Mixed form: complex64
+    factorization at TOL_POW args.mixed_tol preconditioning FP64 GMRES to 1e-6; its time
+    includes its own operator assembly. The plain FP64 path is also run at TOL_POW
+    args.plain_tol_accurate to give a plain solution at comparable accuracy."""
+    rel = lambda r: float(np.linalg.norm(r) / np.linalg.norm(b))
+    log("FP64 residual operator (TOL 10)")
+    t0 = time.perf_counter()
+    op = R.Operator(g, bb, 64, 10)
+    t_op = time.perf_counter() - t0
+    res = {"residual_operator": {"tol_pow": 10, "assembly_s": t_op},
+           "plain_relres": rel(b - op.apply(x_plain))}
+    # mixed: complex64 factorization + FP64 GMRES with its own (TOL 8) operator
+    log(f"mixed: complex64 factorization at TOL {args.mixed_tol} + GMRES")
+    t0 = time.perf_counter()
+    s32 = R.AIFMM(g, bb, 64, args.mixed_tol, args.mixed_tol, 0, R.C64)
+    t1 = time.perf_counter()
+    s32.factorize()
+    t2 = time.perf_counter()
+    opm = R.Operator(g, bb, 64, 8)
+    t3 = time.perf_counter()
+    xm, it, rr = s32.refine(opm, b, tol=1e-6, maxit=60, restart=30)
+    t4 = time.perf_counter()
+    res["mixed"] = {"precision": "complex64 factor + FP64 GMRES (CGS2)", "tol_pow_factor": args.mixed_tol,
+                    "tol_pow_operator": 8, "assembly_s": t1 - t0, "factor_s": t2 - t1, "operator_s": t3 - t2,
+                    "refine_s": t4 - t3, "gmres_iterations": it, "relres_gmres": rr,
+                    "relres_tol10": rel(b - op.apply(xm)), "total_s": t4 - t0}
+    del s32, opm
+    # plain FP64 at a tolerance that reaches the same accuracy class
+    tp = args.plain_tol_accurate
+    log(f"plain FP64 at TOL {tp}")
+    t0 = time.perf_counter()
+    sa = R.AIFMM(g, bb, 64, tp, tp, 0, R.C128)
+    sa.factorize()
+    xa = sa.solve(b)
+    t5 = time.perf_counter() - t0
+    res["plain_accurate"] = {"tol_pow": tp, "total_s": t5, "relres_tol10": rel(b - op.apply(xa))}
+    if res["mixed"]["relres_tol10"] <= 1e-6 and res["plain_accurate"]["relres_tol10"] <= 1e-6:
+        res["speedup_mixed_vs_plain_at_1e-6"] = t5 / res["mixed"]["total_s"]
+    return res
 
 
 if __name__ == "__main__":
