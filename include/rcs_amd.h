@@ -106,6 +106,35 @@ int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y);
 int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int nrhs, double* x, double tol, int maxit,
                       int restart, int* iters, double* relres);
 
+/* Learned ACA rank predictor. No counterpart in the reference: its ACA (src/ACA.cpp:639-859)
+ * only stops adaptively, one step at a time. An MLP (8 inputs, 2 x 64 GELU, 1 output =
+ * log(1 + rank); BF16 weights, FP32 accumulate) with HIP forward, backward and Adam kernels.
+ * params: 4801 floats (W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3); mean/scale: the
+ * 8-feature standardisation. X: n x 8 floats, row-major; t: targets log(1 + rank). */
+typedef struct rcs_mlp_s* rcs_mlp_t;
+int rcs_mlp_create(unsigned seed, rcs_mlp_t* out);
+void rcs_mlp_destroy(rcs_mlp_t m);
+int rcs_mlp_get(rcs_mlp_t m, float* params, float* mean, float* scale);
+int rcs_mlp_set(rcs_mlp_t m, const float* params, const float* mean, const float* scale);
+int rcs_mlp_forward(rcs_mlp_t m, const float* X, int n, float* y);
+/* loss = mean (y - t)^2 and d loss / d params (4801 floats) at the BF16 weights. */
+int rcs_mlp_gradient(rcs_mlp_t m, const float* X, const float* t, int n, float* grad, double* loss);
+/* Full-batch Adam (iters steps, learning rate lr) after standardising with X's statistics. */
+int rcs_mlp_train(rcs_mlp_t m, const float* X, const float* t, int n, int iters, float lr, double* loss);
+/* Solvers created after this call predict every ACA rank with m (NULL: none) to size the
+ * ACA buffers and launch that many steps before the first host check; the adaptive stop
+ * stays the judge (a box keeps iterating past a low prediction). */
+int rcs_set_rank_model(rcs_mlp_t m);
+/* ACA steps launched between host checks once the predicted pass is over (default 1, the
+ * reference's one step per check). Skeletons do not depend on it. */
+int rcs_set_aca_burst(int steps);
+/* Rank log of the assembly: 12 floats per ACA problem (8 predictor features, rank found,
+ * rank predicted or 0, level, 0 incoming / 1 outgoing). Writes up to cap_rows rows (out may
+ * be NULL) and returns the number of rows. */
+int rcs_solver_rank_log(rcs_solver_t s, float* out, int cap_rows);
+/* ACA step launches, host checks and seconds spent in the assembly's ACA. */
+int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, double* seconds);
+
 /* Statistics (the driver's prints, test/sphere.cpp:217-262):
  * info[0..4] = nLevels, startLevel, n_rrqr, n_compress_p2p, n_compress_p2l;
  * vals[0..10] = t_assemble, t_factor, t_solve, gemm_flops, getrf_flops, getrs_flops, qr_flops,

@@ -59,6 +59,18 @@ def lib():
         L.rcs_operator_destroy.argtypes = [vp]
         L.rcs_operator_apply.argtypes = [vp, dp, C.c_int, dp]
         L.rcs_solver_refine.argtypes = [vp, vp, dp, C.c_int, dp, C.c_double, C.c_int, C.c_int, ip, dp]
+        fp = C.POINTER(C.c_float)
+        L.rcs_mlp_create.argtypes = [C.c_uint, C.POINTER(vp)]
+        L.rcs_mlp_destroy.argtypes = [vp]
+        L.rcs_mlp_get.argtypes = [vp, fp, fp, fp]
+        L.rcs_mlp_set.argtypes = [vp, fp, fp, fp]
+        L.rcs_mlp_forward.argtypes = [vp, fp, C.c_int, fp]
+        L.rcs_mlp_gradient.argtypes = [vp, fp, fp, C.c_int, fp, dp]
+        L.rcs_mlp_train.argtypes = [vp, fp, fp, C.c_int, C.c_int, C.c_float, dp]
+        L.rcs_set_rank_model.argtypes = [vp]
+        L.rcs_set_aca_burst.argtypes = [C.c_int]
+        L.rcs_solver_rank_log.argtypes = [vp, fp, C.c_int]
+        L.rcs_solver_aca_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
         _LIB = L
     return _LIB
 
@@ -69,6 +81,8 @@ EXPORTED = [
     "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
     "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
     "rcs_operator_create", "rcs_operator_destroy", "rcs_operator_apply", "rcs_solver_refine",
+    "rcs_mlp_create", "rcs_mlp_destroy", "rcs_mlp_get", "rcs_mlp_set", "rcs_mlp_forward", "rcs_mlp_gradient",
+    "rcs_mlp_train", "rcs_set_rank_model", "rcs_set_aca_burst", "rcs_solver_rank_log", "rcs_solver_aca_stats",
 ]
 
 PROFILE_TAGS = ["schur", "getrf", "getrs", "entries", "aca", "rrqr", "update"]
@@ -288,6 +302,87 @@ class AIFMM:
                                        _ip(it), _dp(rr)))
         X = X.reshape((self.N, nrhs), order="F")
         return (X[:, 0] if single else X), int(it[0]), float(rr[0])
+
+
+    def rank_log(self):
+        """One row per ACA problem of the assembly: 8 predictor features, rank found,
+        rank predicted (0 without a model), level, 0 incoming / 1 outgoing."""
+        n = lib().rcs_solver_rank_log(self.h, None, 0)
+        if n < 0:
+            _check(n)
+        out = np.zeros((max(n, 1), 12), dtype=np.float32)
+        lib().rcs_solver_rank_log(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        return out[:n]
+
+    def aca_stats(self):
+        st, sy, sec = C.c_longlong(), C.c_longlong(), C.c_double()
+        _check(lib().rcs_solver_aca_stats(self.h, C.byref(st), C.byref(sy), C.byref(sec)))
+        return {"steps": st.value, "syncs": sy.value, "seconds": sec.value}
+
+
+MLP_NP = 4801  # W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class RankMLP:
+    """Learned ACA rank predictor (HIP forward/backward/Adam kernels, BF16 weights)."""
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        _check(lib().rcs_mlp_create(seed, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rcs_mlp_destroy(self.h)
+            self.h = None
+
+    def get(self):
+        p = np.zeros(MLP_NP, dtype=np.float32)
+        m = np.zeros(8, dtype=np.float32)
+        s = np.zeros(8, dtype=np.float32)
+        _check(lib().rcs_mlp_get(self.h, _fp(p), _fp(m), _fp(s)))
+        return p, m, s
+
+    def set(self, params, mean, scale):
+        p, m, s = (np.ascontiguousarray(a, dtype=np.float32) for a in (params, mean, scale))
+        _check(lib().rcs_mlp_set(self.h, _fp(p), _fp(m), _fp(s)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        y = np.zeros(len(X), dtype=np.float32)
+        _check(lib().rcs_mlp_forward(self.h, _fp(X), len(X), _fp(y)))
+        return y
+
+    def gradient(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_NP, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().rcs_mlp_gradient(self.h, _fp(X), _fp(t), len(X), _fp(g), C.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, t, iters=2000, lr=3e-3):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().rcs_mlp_train(self.h, _fp(X), _fp(t), len(X), iters, lr, C.byref(loss)))
+        return loss.value
+
+    def predict_ranks(self, X):
+        return np.maximum(1, np.ceil(np.expm1(self.forward(X).astype(np.float64)))).astype(int)
+
+
+def set_rank_model(model):
+    """Use `model` (a RankMLP, or None) for the ACA of solvers created afterwards."""
+    _check(lib().rcs_set_rank_model(model.h if model is not None else None))
+
+
+def set_aca_burst(steps):
+    _check(lib().rcs_set_aca_burst(int(steps)))
 
 
 class Operator:

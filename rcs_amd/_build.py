@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librcs_amd.so")
-SOURCES = ["entries.hip", "dense.hip", "rrqr.hip", "aca.hip", "tree.cpp", "refine.cpp", "capi.cpp"]
+SOURCES = ["entries.hip", "dense.hip", "rrqr.hip", "aca.hip", "mlp.hip", "tree.cpp", "refine.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -45,6 +45,9 @@ template <typename R, typename T>
 __global__ void __launch_bounds__(256) aca_long_kernel(DevGeometry g, int kind, AcaBox<T>* boxes, const int2* work) {
   const int b = work[blockIdx.x].x, chunk = work[blockIdx.x].y;
   AcaBox<T>& bx = boxes[b];
+  // a converged box, or one whose U/V capacity is full, takes no step (bursts of steps run
+  // between host checks, so the launch lists can be stale)
+  if (!bx.active || bx.rank >= bx.cap) return;
   const int rank = bx.rank, s = bx.s_cur, nl = bx.nl;
   __shared__ T us[256];
   __shared__ double rv[256];
@@ -114,6 +117,11 @@ __global__ void __launch_bounds__(256) aca_long_kernel(DevGeometry g, int kind, 
 template <typename T>
 __global__ void __launch_bounds__(256) aca_pivot_kernel(AcaBox<T>* boxes, const int* ids) {
   AcaBox<T>& bx = boxes[ids[blockIdx.x]];
+  if (!bx.active) return;
+  if (bx.rank >= bx.cap) {  // capacity full: skip the step (the host grows U/V, then resumes)
+    if (threadIdx.x == 0) bx.retry = 1;
+    return;
+  }
   __shared__ double rv[256];
   __shared__ int ri[256];
   const int tid = threadIdx.x;
@@ -319,7 +327,7 @@ __global__ void fill_u8_kernel(unsigned char* p, size_t n, unsigned char v) {
 
 template <typename T>
 void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
-                 std::vector<AcaResult>& out, Ctx& cx) {
+                 std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats) {
   using R = decltype(T().re);
   const int nb = (int)probs.size();
   out.assign(nb, AcaResult());
@@ -342,7 +350,8 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
     x.branch = (x.N1 < x.N2) ? 1 : 2;
     x.ns = std::min(x.N1, x.N2);
     x.nl = std::max(x.N1, x.N2);
-    x.cap = std::min(x.ns, 64);
+    // U/V capacity: the predicted rank with a margin when there is one
+    x.cap = P.pred > 0 ? std::min(x.ns, P.pred + P.pred / 4 + 8) : std::min(x.ns, 64);
     x.nchunks = (x.nl + ACA_CHUNK - 1) / ACA_CHUNK;
     int* dr = (int*)alloc(sizeof(int) * x.N1);
     int* dc = (int*)alloc(sizeof(int) * x.N2);
@@ -368,18 +377,35 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
   }
   AcaBox<T>* d_boxes = (AcaBox<T>*)cx.pool.alloc(sizeof(AcaBox<T>) * nb);
   cx.upload(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb);
-  struct Hdr { int rank, active, retry, cap; };
   std::vector<int> active = live;
+  bool predicted = false;
+  for (int b : live) predicted = predicted || probs[b].pred > 0;
+  bool first = true;
   while (!active.empty()) {
     std::vector<int2> work;
     for (int b : active)
       for (int c = 0; c < hb[b].nchunks; ++c) work.push_back(make_int2(b, c));
+    // steps between host checks: the predicted remaining rank on the first pass, then the
+    // fixed burst (converged boxes skip the extra steps inside the kernels, so a burst
+    // never changes the skeletons, only how often the host looks)
+    int burst = std::max(1, stats.burst);
+    if (predicted && first) {
+      int need = 1;
+      for (int b : active) need = std::max(need, std::min(probs[b].pred, hb[b].cap) - hb[b].rank);
+      burst = std::min(need, 256);
+    }
+    first = false;
     const int2* d_work = (const int2*)cx.push(work.data(), sizeof(int2) * work.size());
     const int* d_ids = (const int*)cx.push(active.data(), sizeof(int) * active.size());
-    hipLaunchKernelGGL((aca_long_kernel<R, T>), dim3(work.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_work);
-    hipLaunchKernelGGL(aca_pivot_kernel<T>, dim3(active.size()), dim3(256), 0, cx.st, d_boxes, d_ids);
-    hipLaunchKernelGGL(aca_norm_kernel<T>, dim3(work.size()), dim3(256), 0, cx.st, d_boxes, d_work);
-    hipLaunchKernelGGL((aca_short_kernel<R, T>), dim3(active.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_ids, tol);
+    for (int st = 0; st < burst; ++st) {
+      hipLaunchKernelGGL((aca_long_kernel<R, T>), dim3(work.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_work);
+      hipLaunchKernelGGL(aca_pivot_kernel<T>, dim3(active.size()), dim3(256), 0, cx.st, d_boxes, d_ids);
+      hipLaunchKernelGGL(aca_norm_kernel<T>, dim3(work.size()), dim3(256), 0, cx.st, d_boxes, d_work);
+      hipLaunchKernelGGL((aca_short_kernel<R, T>), dim3(active.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_ids,
+                         tol);
+    }
+    stats.steps += burst;
+    stats.syncs += 1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(hb.data(), d_boxes, sizeof(AcaBox<T>) * nb, hipMemcpyDeviceToHost, cx.st));
     cx.sync();
@@ -462,9 +488,9 @@ void max_col_norms(const std::vector<DMat<T>>& mats, std::vector<double>& out, C
 }
 
 template void aca_batched<cpx<double>>(const DevGeometry&, int, double, const std::vector<AcaProblem>&,
-                                       std::vector<AcaResult>&, Ctx&);
+                                       std::vector<AcaResult>&, Ctx&, AcaStats&);
 template void aca_batched<cpx<float>>(const DevGeometry&, int, double, const std::vector<AcaProblem>&,
-                                      std::vector<AcaResult>&, Ctx&);
+                                      std::vector<AcaResult>&, Ctx&, AcaStats&);
 template void max_col_norms<cpx<double>>(const std::vector<DMat<cpx<double>>>&, std::vector<double>&, Ctx&);
 template void max_col_norms<cpx<float>>(const std::vector<DMat<cpx<float>>>&, std::vector<double>&, Ctx&);
 

@@ -10,6 +10,7 @@
 #include "dense.hpp"
 #include "entries.hpp"
 #include "geometry.hpp"
+#include "mlp.hpp"
 #include "refine.hpp"
 #include "tree.hpp"
 
@@ -26,6 +27,13 @@ struct rcs_operator_s {
   std::unique_ptr<SolverBase> s;  // assembled, never factorized
   std::shared_ptr<DeviceGeometry> g;
 };
+struct rcs_mlp_s {
+  std::unique_ptr<RankMLP> m;
+};
+namespace {
+rcs_mlp_t g_rank_model = nullptr;  // used by solvers created afterwards
+int g_aca_burst = 1;
+}  // namespace
 
 namespace {
 thread_local std::string g_err;
@@ -121,7 +129,8 @@ void aca_t(rcs_mesh_t m, double tol, const int* rows, int nr, const int* cols, i
   probs[0].rows.assign(rows, rows + nr);
   probs[0].cols.assign(cols, cols + nc);
   std::vector<AcaResult> res;
-  aca_batched<T>(m->g->d, m->g->h.kind, tol, probs, res, cx);
+  AcaStats st;
+  aca_batched<T>(m->g->d, m->g->h.kind, tol, probs, res, cx, st);
   *rank = res[0].rank;
   for (int i = 0; i < res[0].rank; ++i) {
     rb[i] = probs[0].rows.empty() ? 0 : res[0].row_bases[i];
@@ -357,11 +366,77 @@ int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_p
   p.fillin_tol = fillin_tol;
   p.fillin_rank = fillin_rank;
   p.variant = variant;
+  p.rank_model = g_rank_model ? g_rank_model->m.get() : nullptr;
+  p.aca_burst = g_aca_burst;
   std::unique_ptr<rcs_solver_s> s(new rcs_solver_s);
   s->g = m->g;
   s->s = make_solver(m->g, p, precision, ctx());
   s->s->assemble();
   *out = s.release();
+  return 0;
+  CATCH
+}
+int rcs_solver_rank_log(rcs_solver_t s, float* out, int cap_rows) {
+  TRY const std::vector<float>& v = s->s->rank_log();
+  const int rows = (int)(v.size() / RANK_LOG_COLS);
+  if (out) std::memcpy(out, v.data(), sizeof(float) * RANK_LOG_COLS * std::min(rows, std::max(cap_rows, 0)));
+  return rows;
+  CATCH
+}
+int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, double* seconds) {
+  TRY SolverStats st = s->s->stats();
+  if (steps) *steps = st.aca_steps;
+  if (syncs) *syncs = st.aca_syncs;
+  if (seconds) *seconds = st.t_aca;
+  return 0;
+  CATCH
+}
+int rcs_mlp_create(unsigned seed, rcs_mlp_t* out) {
+  TRY std::unique_ptr<rcs_mlp_s> m(new rcs_mlp_s);
+  m->m.reset(new RankMLP(ctx(), seed));
+  *out = m.release();
+  return 0;
+  CATCH
+}
+void rcs_mlp_destroy(rcs_mlp_t m) {
+  if (g_rank_model == m) g_rank_model = nullptr;
+  delete m;
+}
+int rcs_mlp_get(rcs_mlp_t m, float* params, float* mean, float* scale) {
+  TRY m->m->get(params, mean, scale);
+  return 0;
+  CATCH
+}
+int rcs_mlp_set(rcs_mlp_t m, const float* params, const float* mean, const float* scale) {
+  TRY m->m->set(params, mean, scale);
+  return 0;
+  CATCH
+}
+int rcs_mlp_forward(rcs_mlp_t m, const float* X, int n, float* y) {
+  TRY m->m->forward_host(X, n, y);
+  return 0;
+  CATCH
+}
+int rcs_mlp_gradient(rcs_mlp_t m, const float* X, const float* t, int n, float* grad, double* loss) {
+  TRY double l = m->m->gradient_host(X, t, n, grad);
+  if (loss) *loss = l;
+  return 0;
+  CATCH
+}
+int rcs_mlp_train(rcs_mlp_t m, const float* X, const float* t, int n, int iters, float lr, double* loss) {
+  TRY double l = m->m->train_host(X, t, n, iters, lr);
+  if (loss) *loss = l;
+  return 0;
+  CATCH
+}
+int rcs_set_rank_model(rcs_mlp_t m) {
+  TRY g_rank_model = m;
+  return 0;
+  CATCH
+}
+int rcs_set_aca_burst(int steps) {
+  TRY if (steps < 1) throw std::runtime_error("burst must be at least 1");
+  g_aca_burst = steps;
   return 0;
   CATCH
 }

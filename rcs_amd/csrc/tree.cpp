@@ -3,6 +3,8 @@
 // takes the same decisions (ranks, fill-in pairs, compression order) as the reference.
 #include "tree.hpp"
 
+#include "mlp.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -231,6 +233,9 @@ class Solver : public SolverBase {
     h.locations(loc);
     px.resize(N); py.resize(N); pz.resize(N);
     for (int i = 0; i < N; ++i) { px[i] = loc[3 * i]; py[i] = loc[3 * i + 1]; pz[i] = loc[3 * i + 2]; }
+    gcx = gcy = gcz = 0;
+    for (int i = 0; i < N; ++i) { gcx += px[i]; gcy += py[i]; gcz += pz[i]; }
+    if (N > 0) { gcx /= N; gcy /= N; gcz /= N; }
     TOL_POW = P.tol_pow;
     fill_in_TOL_POW = P.fillin_tol;
     fillInTolerance = std::pow(10.0, -1.0 * TOL_POW);
@@ -333,8 +338,61 @@ class Solver : public SolverBase {
       out.insert(out.end(), v.begin(), v.end());
     }
   }
-  void getNodes() {
+  // The rank predictor's inputs for box k at level j (an ACA problem with `own` points on
+  // the box side): mean interaction-list centre distance / box side, kappa * box side,
+  // log10 tolerance, level, direction cosines of the box centre about the point cloud's
+  // centroid, log2(1 + own).
+  void rank_features(int j, int k, int own, float* f) {
+    const Box<T>& b = tree[j][k];
+    double d = 0;
+    for (int l : b.IL) {
+      const Box<T>& o = tree[j][l];
+      d += std::sqrt((b.cx - o.cx) * (b.cx - o.cx) + (b.cy - o.cy) * (b.cy - o.cy) + (b.cz - o.cz) * (b.cz - o.cz));
+    }
+    f[0] = b.IL.empty() ? 0.0f : (float)(d / b.IL.size() / side[j]);
+    f[1] = G->h.kind == KIND_EFIE ? (float)(G->d.kappa * side[j]) : 0.0f;
+    f[2] = (float)(-TOL_POW);
+    f[3] = (float)j;
+    double ux = b.cx - gcx, uy = b.cy - gcy, uz = b.cz - gcz, un = std::sqrt(ux * ux + uy * uy + uz * uz);
+    if (un > 0) { ux /= un; uy /= un; uz /= un; }
+    f[4] = (float)ux; f[5] = (float)uy; f[6] = (float)uz;
+    f[7] = (float)std::log2(1.0 + own);
+  }
+  // Adaptive ACA of one level's problems (incoming: inout 0, outgoing: 1), with predicted
+  // ranks when a model is set, and the rank log.
+  void run_aca(int j, int inout, std::vector<AcaProblem>& probs, const std::vector<int>& which,
+               std::vector<AcaResult>& res) {
     const double tol = std::pow(10.0, -TOL_POW);
+    std::vector<float> feat(which.size() * MLP_IN);
+    for (size_t i = 0; i < which.size(); ++i) {
+      const AcaProblem& p = probs[which[i]];
+      rank_features(j, which[i], (int)(inout == 0 ? p.rows.size() : p.cols.size()), &feat[i * MLP_IN]);
+    }
+    std::vector<int> pred(which.size(), 0);
+    if (P.rank_model && !which.empty()) {
+      P.rank_model->predict_ranks(feat.data(), (int)which.size(), pred.data());
+      for (size_t i = 0; i < which.size(); ++i) probs[which[i]].pred = pred[i];
+    }
+    AcaStats ast;
+    ast.burst = P.aca_burst;
+    double t0 = wall();
+    int h = cx.prof.begin(cx.st);
+    aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx, ast);
+    cx.prof.end(cx.st, h, PROF_ACA, 0.0);
+    stats_.t_aca += wall() - t0;
+    stats_.aca_steps += ast.steps;
+    stats_.aca_syncs += ast.syncs;
+    for (size_t i = 0; i < which.size(); ++i) {
+      rank_log_.insert(rank_log_.end(), feat.begin() + i * MLP_IN, feat.begin() + (i + 1) * MLP_IN);
+      rank_log_.push_back((float)res[which[i]].rank);
+      rank_log_.push_back((float)pred[i]);
+      rank_log_.push_back((float)j);
+      rank_log_.push_back((float)inout);
+    }
+  }
+  const std::vector<float>& rank_log() const override { return rank_log_; }
+
+  void getNodes() {
     for (int j = nLevels; j >= startLevel; --j) {
       const int nb = nBoxes[j];
       // incoming (T:1155-1256)
@@ -350,9 +408,7 @@ class Solver : public SolverBase {
           if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
         }
         std::vector<AcaResult> res;
-        int h = cx.prof.begin(cx.st);
-        aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
-        cx.prof.end(cx.st, h, PROF_ACA, 0.0);
+        run_aca(j, 0, probs, which, res);
         trace("level %d incoming skeletons (%d boxes)", j, (int)which.size());
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
@@ -380,9 +436,7 @@ class Solver : public SolverBase {
           if (probs[k].rows.empty() || probs[k].cols.empty()) probs[k].rows.clear(), probs[k].cols.clear();
         }
         std::vector<AcaResult> res;
-        int h = cx.prof.begin(cx.st);
-        aca_batched<T>(G->d, G->h.kind, tol, probs, res, cx);
-        cx.prof.end(cx.st, h, PROF_ACA, 0.0);
+        run_aca(j, 1, probs, which, res);
         trace("level %d outgoing skeletons (%d boxes)", j, (int)which.size());
         std::vector<const std::vector<int>*> rr, cc;
         std::vector<DMat<T>*> outs;
@@ -1565,6 +1619,8 @@ class Solver : public SolverBase {
   int rankVec[32];
   std::vector<int> nBoxes;
   std::vector<double> side, px, py, pz;
+  double gcx = 0, gcy = 0, gcz = 0;  // centroid of the unknown locations
+  std::vector<float> rank_log_;
   std::vector<std::vector<Box<T>>> tree;
   std::vector<std::pair<int, int>> P2Plist, P2L_M2P;
   DMat<T> luBase;

@@ -19,10 +19,14 @@
 
 namespace rcs {
 
+class RankMLP;
+
 struct SolverParams {
   double bbox[6];
   int leafSize = 64, tol_pow = 6, fillin_tol = 6, fillin_rank = 0;
   int variant = 0;  // 0: memory-optimized tree (reference default); 1: AIFMMTree2D_3D parameters
+  RankMLP* rank_model = nullptr;  // predicted ACA ranks (sizes and step bursts); null: none
+  int aca_burst = 1;              // ACA steps between host checks after the predicted pass
 };
 
 struct SolverStats {
@@ -33,7 +37,14 @@ struct SolverStats {
   double t_getrf = 0, t_schur = 0, t_compress = 0;
   size_t peak_bytes = 0;
   double factor_bytes = 0;  // bytes of the stored factor read once by a solve
+  long long aca_steps = 0, aca_syncs = 0;  // ACA step launches and host checks of the assembly
+  double t_aca = 0;
 };
+
+// One ACA problem of the skeleton construction: the predictor's 8 features, the rank the
+// adaptive ACA found, the rank predicted (0 without a model), the level, 0 incoming / 1
+// outgoing.
+constexpr int RANK_LOG_COLS = 12;
 
 class DeviceGeometry {
  public:
@@ -59,6 +70,8 @@ class SolverBase {
   virtual void apply_device(const void* x, int nrhs, void* y) = 0;
   virtual int size() const = 0;
   virtual Ctx& context() = 0;
+  // rows of RANK_LOG_COLS floats, one per ACA problem of the assembly
+  virtual const std::vector<float>& rank_log() const = 0;
   virtual SolverStats stats() const = 0;
   virtual void ranks(int level, std::vector<int>& out) = 0;
   // Copy one stored block to the host as complex128 (which: 0 L2P, 1 P2M, 2 M2L[key],

@@ -299,3 +299,72 @@ def test_refinement_reaches_dense_solution(prec, tp_factor):
           f"after {it} GMRES iterations (relres {rr:.1e})")
     assert rr <= 1e-10
     assert _rel(x, xd) < 1e-8 and _rel(x, xd) < 1e-2 * _rel(x0, xd)
+
+
+def test_rank_mlp_matches_cpu_oracle(rng):
+    """HIP forward and backward of the rank MLP against the numpy oracle (same BF16 weights)."""
+    from oracle import rank_mlp as Q
+    net = R.RankMLP(seed=3)
+    p = (rng.normal(scale=0.2, size=R.MLP_NP)).astype(np.float32)
+    X = rng.normal(size=(300, 8)).astype(np.float32)
+    t = rng.normal(size=300).astype(np.float32)
+    mean, scale = X.mean(0).astype(np.float32), X.std(0).astype(np.float32)
+    net.set(p, mean, scale)
+    y = net.forward(X)
+    yo = Q.forward(p, mean, scale, X)
+    assert np.max(np.abs(y - yo)) < 1e-4 * max(1.0, np.max(np.abs(yo)))
+    loss, g = net.gradient(X, t)
+    lo, go = Q.loss_and_grad(p, mean, scale, X, t)
+    assert abs(loss - lo) < 1e-4 * lo
+    assert np.linalg.norm(g - go) < 1e-4 * np.linalg.norm(go)
+
+
+def _rank_samples(level_list, kinds=(R.EFIE,), tol=6):
+    Xs, ts = [], []
+    for lvl, axes in level_list:
+        nodes, tris = M.ellipsoid(lvl, axes)
+        for kind in kinds:
+            g = R.Mesh(nodes, tris, kind, 1e9)
+            s = R.AIFMM(g, M.bounding_box(g.locations()), 64, tol, tol, 0, R.C128)
+            log = s.rank_log()
+            Xs.append(log[:, :8])
+            ts.append(np.log1p(log[:, 8]))
+    return np.vstack(Xs), np.concatenate(ts)
+
+
+def test_rank_model_trains_and_keeps_skeletons():
+    """Train the predictor on ACA ranks logged from a few assemblies, then assemble and solve
+    an unseen geometry with it: the skeletons (and so the solution) are unchanged, the ACA
+    needs fewer host checks, and the miss rate (boxes whose rank exceeded the prediction)
+    is reported."""
+    X, t = _rank_samples([(3, (0.3, 0.06, 0.06)), (3, (0.15, 0.15, 0.15)), (3, (0.2, 0.1, 0.05))],
+                         kinds=(R.EFIE, R.LAPLACE))
+    net = R.RankMLP(seed=11)
+    loss0 = net.gradient(X, t)[0]
+    loss = net.train(X, t, iters=1500, lr=3e-3)
+    assert loss < 0.25 * loss0
+    nodes, tris = M.perturbed_ellipsoid(3, (0.25, 0.08, 0.08), 0.02, seed=5)
+    g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+    bb = M.bounding_box(g.locations())
+    b = g.rhs()
+    s0 = R.AIFMM(g, bb, 64, 6, 6, 0, R.C128)
+    st0 = s0.aca_stats()
+    s0.factorize()
+    x0 = s0.solve(b)
+    R.set_rank_model(net)
+    try:
+        s1 = R.AIFMM(g, bb, 64, 6, 6, 0, R.C128)
+    finally:
+        R.set_rank_model(None)
+    st1 = s1.aca_stats()
+    s1.factorize()
+    x1 = s1.solve(b)
+    log1 = s1.rank_log()
+    assert np.array_equal(s0.rank_log()[:, 8], log1[:, 8])  # same ACA ranks
+    assert _rel(x1, x0) < 1e-13
+    miss = float(np.mean(log1[:, 8] > log1[:, 9]))
+    rel = np.abs(log1[:, 9] - log1[:, 8]) / np.maximum(log1[:, 8], 1)
+    print(f"samples {len(X)}, loss {loss0:.3f} -> {loss:.4f}; unseen geometry: median |pred-rank|/rank "
+          f"{np.median(rel):.2f}, miss rate {miss:.2f}; ACA host checks {st0['syncs']} -> {st1['syncs']}")
+    assert st1["syncs"] < st0["syncs"]
+    assert np.median(rel) < 0.5

@@ -220,3 +220,27 @@ def test_aifmm_efie_with_compressions():
     (e5, i5, r5), = _aifmm_vs_dense(O.EFIE, 1e9, (5,))
     assert i5["n_rrqr"] > 0 and i5["n_compress_p2p"] + i5["n_compress_p2l"] > 0
     assert e5 < 1e-2
+
+
+def test_rank_mlp_oracle_gradient_finite_differences():
+    """The MLP oracle's analytic gradient against central differences of its loss (float64,
+    without BF16 rounding so that the parameters can be perturbed continuously)."""
+    from oracle import rank_mlp as Q
+    rng = np.random.default_rng(7)
+    p = rng.normal(scale=0.3, size=Q.NP)
+    X = rng.normal(size=(40, 8))
+    t = rng.normal(size=40)
+    mean, scale = X.mean(0), X.std(0)
+    loss, g = Q.loss_and_grad(p, mean, scale, X, t, bf16=False)
+    for idx in rng.choice(Q.NP, 40, replace=False):
+        e = np.zeros(Q.NP)
+        e[idx] = 1e-6
+        lp, _ = Q.loss_and_grad(p + e, mean, scale, X, t, bf16=False)
+        lm, _ = Q.loss_and_grad(p - e, mean, scale, X, t, bf16=False)
+        assert abs((lp - lm) / 2e-6 - g[idx]) < 1e-6 * max(1.0, abs(g[idx]))
+    # GELU closed form: gelu(0) = 0, gelu(1) = Phi(1) = 0.8413447460685429, gelu'(0) = 0.5
+    assert abs(Q.gelu(np.array([1.0]))[0] - 0.8413447460685429) < 1e-15
+    assert Q.gelu(np.array([0.0]))[0] == 0.0 and abs(Q.gelu_d(np.array([0.0]))[0] - 0.5) < 1e-15
+    # BF16 rounding: 1 + 2^-8 is a tie (rounds to even 1.0); 1 + 3*2^-9 rounds up to 1 + 2^-7
+    assert Q.bf16_round(np.array([1.0 + 2 ** -8], np.float32))[0] == 1.0
+    assert Q.bf16_round(np.array([1.0 + 3 * 2 ** -9], np.float32))[0] == np.float32(1.0 + 2 ** -7)
