@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=3)
     ap.add_argument("--no-mixed", action="store_true", help="skip the accuracy / mixed-precision comparison")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank training and ablation")
     ap.add_argument("--mixed-tol", type=int, default=4, help="TOL_POW of the complex64 factorization")
     ap.add_argument("--plain-tol-accurate", type=int, default=9, help="TOL_POW of the accurate plain FP64 run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -207,12 +208,59 @@ def main():
                                 "work": v["flops"] / args.steps} for k, v in prof.items()},
         "rcs_theta0_dbsm": float(10 * np.log10(rcs[0])),
     }
+    own_log = s.rank_log()
+    del s
     if not args.no_mixed:
-        del s
         out.update(accuracy_and_mixed(R, M, g, bb, b, x, freq, args, log))
+    if not args.no_learned:
+        out["learned_rank"] = learned_rank(R, M, own_log, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
     print(json.dumps(out))
+
+
+def learned_rank(R, M, own_log, log):
+    """Train the rank predictor on ACA ranks logged from the Laplace sphere, the EFIE sphere
+    (including this benchmark's own assembly) and seeded ellipsoids, then assemble the
+    perturbed-ellipsoid workload (122,880 unknowns, 2 % radial noise, ka = 12) without and
+    with it: ACA host checks and time, assembly time, miss rate (boxes whose rank exceeded
+    the prediction), median relative rank error, and whether the skeletons are identical."""
+    from rcs_amd import rank_model as RM
+    log("learned rank: collecting and training")
+    t0 = time.perf_counter()
+    X, t = RM.collect(RM.training_configs())
+    X = np.vstack([X, own_log[:, :8]])
+    t = np.concatenate([t, np.log1p(own_log[:, 8])]).astype(np.float32)
+    t1 = time.perf_counter()
+    net, loss0, loss = RM.train(X, t, seed=1, iters=2000)
+    t2 = time.perf_counter()
+    freq = 3.82e9
+    nodes, tris = M.perturbed_ellipsoid(6, (0.15, 0.12, 0.10), 0.02, seed=20261203)
+    g = R.Mesh(nodes, tris, R.EFIE, freq)
+    bb = M.bounding_box(g.locations())
+    res = {"training_samples": int(len(X)), "collect_s": t1 - t0, "train_s": t2 - t1, "train_iters": 2000,
+           "loss_initial": loss0, "loss_final": loss,
+           "workload": f"perturbed ellipsoid (0.15, 0.12, 0.10) m, 2% radial noise, N={g.N}, f={freq:g} Hz, TOL 6"}
+    runs = {}
+    for name, model in (("adaptive", None), ("learned", net)):
+        log(f"learned rank: assembling the perturbed ellipsoid ({name})")
+        R.set_rank_model(model)
+        try:
+            ta = time.perf_counter()
+            s = R.AIFMM(g, bb, 64, 6, 6, 0, R.C128)
+            ta = time.perf_counter() - ta
+        finally:
+            R.set_rank_model(None)
+        runs[name] = (s.aca_stats(), ta, s.rank_log())
+        del s
+    (a0, ta0, l0), (a1, ta1, l1) = runs["adaptive"], runs["learned"]
+    rel = np.abs(l1[:, 9] - l1[:, 8]) / np.maximum(l1[:, 8], 1)
+    res.update({"aca_host_checks": [a0["syncs"], a1["syncs"]], "aca_s": [a0["seconds"], a1["seconds"]],
+                "assembly_s": [ta0, ta1], "miss_rate": float(np.mean(l1[:, 8] > l1[:, 9])),
+                "median_rel_rank_error": float(np.median(rel)),
+                "skeleton_ranks_identical": bool(np.array_equal(l0[:, 8], l1[:, 8])),
+                "aca_speedup": a0["seconds"] / max(a1["seconds"], 1e-12)})
+    return res
 
 
 def accuracy_and_mixed(R, M, g, bb, b, x_plain, freq, args, log):

@@ -326,8 +326,44 @@ __global__ void fill_u8_kernel(unsigned char* p, size_t n, unsigned char v) {
 }
 
 template <typename T>
+static void aca_group(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
+                      std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats);
+
+// The residual vectors of every box of a group stay in HBM until the group converges
+// (long side x rank each), so the boxes of a level are processed in groups whose expected
+// footprint stays under a budget.
+template <typename T>
 void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
                  std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats) {
+  const size_t budget = size_t(24) << 30;
+  out.assign(probs.size(), AcaResult());
+  std::vector<size_t> grp;
+  size_t used = 0;
+  auto flush = [&]() {
+    if (grp.empty()) return;
+    std::vector<AcaProblem> sub(grp.size());
+    for (size_t i = 0; i < grp.size(); ++i) sub[i] = probs[grp[i]];
+    std::vector<AcaResult> r;
+    aca_group<T>(g, kind, tol, sub, r, cx, stats);
+    for (size_t i = 0; i < grp.size(); ++i) out[grp[i]] = std::move(r[i]);
+    grp.clear();
+    used = 0;
+  };
+  for (size_t b = 0; b < probs.size(); ++b) {
+    const size_t n1 = probs[b].rows.size(), n2 = probs[b].cols.size();
+    const size_t ns = std::min(n1, n2), nl = std::max(n1, n2);
+    const size_t cap = std::min<size_t>(ns, probs[b].pred > 0 ? probs[b].pred + probs[b].pred / 4 + 8 : 256);
+    const size_t bytes = (nl + ns) * cap * sizeof(T);
+    if (!grp.empty() && used + bytes > budget) flush();
+    grp.push_back(b);
+    used += bytes;
+  }
+  flush();
+}
+
+template <typename T>
+static void aca_group(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
+                      std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats) {
   using R = decltype(T().re);
   const int nb = (int)probs.size();
   out.assign(nb, AcaResult());
@@ -389,9 +425,12 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
     // fixed burst (converged boxes skip the extra steps inside the kernels, so a burst
     // never changes the skeletons, only how often the host looks)
     int burst = std::max(1, stats.burst);
-    if (predicted && first) {
+    if (predicted && first) {  // the prediction plus an eighth: extra steps of a converged box are no-ops
       int need = 1;
-      for (int b : active) need = std::max(need, std::min(probs[b].pred, hb[b].cap) - hb[b].rank);
+      for (int b : active) {
+        const int p = probs[b].pred + probs[b].pred / 8 + 2;
+        need = std::max(need, std::min(p, hb[b].cap) - hb[b].rank);
+      }
       burst = std::min(need, 256);
     }
     first = false;

@@ -60,6 +60,7 @@ class DevPool {
   }
   size_t in_use() const { return in_use_; }
   size_t peak() const { return peak_; }
+  void reset_peak() { peak_ = in_use_; }
   size_t total() const { return total_; }
 
  private:

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <unordered_map>
 
 namespace rcs {
 
@@ -81,23 +82,46 @@ class Solver : public SolverBase {
  public:
   using R = decltype(T().re);
   Solver(std::shared_ptr<DeviceGeometry> g, const SolverParams& p, Ctx& c) : G(g), P(p), cx(c) {}
-  ~Solver() override { cx.sync(); }
+  // Box blocks are plain views; the solver owns every pool block it allocated and hands
+  // all of them back (and the pool's cached blocks to the driver) when it goes away, so a
+  // process can build solvers one after another without holding the earlier ones' HBM.
+  ~Solver() override {
+    cx.sync();
+    for (auto& kv : owned_) cx.pool.free(kv.first);
+    owned_.clear();
+    cx.pool.trim();
+  }
 
   // ------------------------------------------------------------ block helpers
+  void* palloc(size_t bytes) {
+    void* p = cx.pool.alloc(bytes);
+    owned_[p] = bytes;
+    own_in_use_ += bytes;
+    own_peak_ = std::max(own_peak_, own_in_use_);
+    return p;
+  }
+  void pfree(void* p) {
+    if (!p) return;
+    auto it = owned_.find(p);
+    if (it == owned_.end()) throw std::runtime_error("solver: free of a block it does not own");
+    own_in_use_ -= it->second;
+    owned_.erase(it);
+    cx.pool.free(p);
+  }
   DMat<T> newmat(int r, int c) {
     DMat<T> m;
     m.r = r;
     m.c = c;
     m.ld = std::max(r, 1);
-    if ((size_t)r * c > 0) m.p = (T*)cx.pool.alloc(sizeof(T) * (size_t)r * c);
+    if ((size_t)r * c > 0) m.p = (T*)palloc(sizeof(T) * (size_t)r * c);
     return m;
   }
   void drop(DMat<T>& m) {
-    if (m.p) cx.pool.free(m.p);
+    if (m.p) pfree(m.p);
     m = DMat<T>();
   }
   void put(DMat<T>& dst, DMat<T> src) {
-    if (dst.p && dst.p != src.p) cx.pool.free(dst.p);
+    if (dst.p && dst.p != src.p) pfree(dst.p);
     dst = src;
   }
   DMat<T> zeros(int r, int c) {
@@ -158,8 +182,8 @@ class Solver : public SolverBase {
   }
   void factor(DMat<T> A, DMat<T>& lu, int*& ip) {
     put(lu, clone(A));
-    if (ip) cx.pool.free(ip);
-    ip = (int*)cx.pool.alloc(sizeof(int) * 2 * std::max(A.r, 1));  // pivots + permutation
+    if (ip) pfree(ip);
+    ip = (int*)palloc(sizeof(int) * 2 * std::max(A.r, 1));  // pivots + permutation
     double t0 = wall();
     int h = cx.prof.begin(cx.st);
     getrf(lu, ip, cx);
@@ -496,7 +520,7 @@ class Solver : public SolverBase {
           adjoint2d(L2P, X, cx.st);
           put(b.L2P, L2P);
           drop(X); drop(lu); drop(D1h);
-          cx.pool.free(ip);
+          pfree(ip);
         } else {
           put(b.L2P, zeros(b.Ac.r, r));
         }
@@ -511,7 +535,7 @@ class Solver : public SolverBase {
           solve_inplace(lu, ip, X);
           put(b.P2M, X);
           drop(lu);
-          cx.pool.free(ip);
+          pfree(ip);
         } else {
           put(b.P2M, zeros(ro, b.Ar.c));
         }
@@ -592,9 +616,14 @@ class Solver : public SolverBase {
 
   void assemble() override {
     double t0 = wall();
+    cx.pool.reset_peak();
     setup();
     trace("setup: N=%d nLevels=%d", N, nLevels);
     getNodes();
+    // the ACA residual vectors were transient: hand their cached blocks back to the device
+    // (the HIP runtime's own allocations, e.g. kernel scratch, come from the same HBM)
+    cx.sync();
+    cx.pool.trim();
     trace("skeletons done, startLevel=%d", startLevel);
     assemble_M2L();
     free_points();
@@ -1239,7 +1268,7 @@ class Solver : public SolverBase {
     }
     DMat<T> X = newmat(N, nrhs), Xb = newmat(N, nrhs), Yb = newmat(N, nrhs);
     to_T(X, DMat<cpx<double>>{(cpx<double>*)x_dev, N, nrhs, N});
-    int* d_order = (int*)cx.pool.alloc(sizeof(int) * std::max(N, 1));  // lives across many launches
+    int* d_order = (int*)palloc(sizeof(int) * std::max(N, 1));  // lives across many launches
     cx.upload(d_order, order.data(), sizeof(int) * order.size());
     gather_rows(Xb, X, d_order, cx.st);
     // per level contiguous multipole / local buffers, box after box
@@ -1321,7 +1350,7 @@ class Solver : public SolverBase {
     from_T(DMat<cpx<double>>{(cpx<double>*)y_dev, N, nrhs, N}, Y);
     for (int j = startLevel; j <= nLevels; ++j) { drop(mult[j]); drop(loc[j]); }
     drop(X); drop(Xb); drop(Yb); drop(Y);
-    cx.pool.free(d_order);
+    pfree(d_order);
   }
   int size() const override { return N; }
   Ctx& context() override { return cx; }
@@ -1370,9 +1399,9 @@ class Solver : public SolverBase {
     return v;
   }
   void solve(const double* rhs, int nrhs, double* x, double* x_box) override {
-    DMat<cpx<double>> B64{(cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
-    DMat<cpx<double>> X64{(cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
-    DMat<cpx<double>> XB64{x_box ? (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * (size_t)N * nrhs) : nullptr,
+    DMat<cpx<double>> B64{(cpx<double>*)palloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
+    DMat<cpx<double>> X64{(cpx<double>*)palloc(sizeof(cpx<double>) * (size_t)N * nrhs), N, nrhs, N};
+    DMat<cpx<double>> XB64{x_box ? (cpx<double>*)palloc(sizeof(cpx<double>) * (size_t)N * nrhs) : nullptr,
                            N, nrhs, N};
     cx.upload(B64.p, rhs, sizeof(cpx<double>) * (size_t)N * nrhs);
     solve_device(B64.p, nrhs, X64.p, XB64.p);
@@ -1380,9 +1409,9 @@ class Solver : public SolverBase {
     if (x_box)
       HIPCHK(hipMemcpyAsync(x_box, XB64.p, sizeof(cpx<double>) * XB64.size(), hipMemcpyDeviceToHost, cx.st));
     cx.sync();
-    cx.pool.free(B64.p);
-    cx.pool.free(X64.p);
-    if (XB64.p) cx.pool.free(XB64.p);
+    pfree(B64.p);
+    pfree(X64.p);
+    if (XB64.p) pfree(XB64.p);
   }
   // rhs / x / x_box: device complex128, N x nrhs (ld N); x in input order, x_box (may be
   // null) in box order. Stream ordered; no host synchronisation.
@@ -1613,6 +1642,8 @@ class Solver : public SolverBase {
   SolverParams P;
   Ctx& cx;
   SolverStats stats_;
+  std::unordered_map<void*, size_t> owned_;  // pool blocks this solver holds, with their sizes
+  size_t own_in_use_ = 0, own_peak_ = 0;
   bool factorized_ = false;
   int N = 0, nLevels = 0, startLevel = 0, TOL_POW = 6, fill_in_TOL_POW = 6;
   double L = 0, eta = 0, fillInTolerance = 0;
@@ -1662,11 +1693,11 @@ template <>
 void Solver<cpx<float>>::download(DMat<cpx<float>> src, double* host) {
   DMat<cpx<double>> t;
   t.r = src.r; t.c = src.c; t.ld = src.r;
-  t.p = (cpx<double>*)cx.pool.alloc(sizeof(cpx<double>) * src.size());
+  t.p = (cpx<double>*)palloc(sizeof(cpx<double>) * src.size());
   cast_c64_to_c128(t, src, cx.st);
   HIPCHK(hipMemcpyAsync(host, t.p, sizeof(cpx<double>) * src.size(), hipMemcpyDeviceToHost, cx.st));
   cx.sync();
-  cx.pool.free(t.p);
+  pfree(t.p);
 }
 
 }  // namespace
