@@ -128,9 +128,9 @@ int rcs_set_rank_model(rcs_mlp_t m);
 /* ACA steps launched between host checks once the predicted pass is over (default 1, the
  * reference's one step per check). Skeletons do not depend on it. */
 int rcs_set_aca_burst(int steps);
-/* Rank log of the assembly: 12 floats per ACA problem (8 predictor features, rank found,
- * rank predicted or 0, level, 0 incoming / 1 outgoing). Writes up to cap_rows rows (out may
- * be NULL) and returns the number of rows. */
+/* Rank log of the assembly: 14 floats per ACA problem (8 predictor features, rank found,
+ * rank predicted or 0, level, 0 incoming / 1 outgoing, short and long side lengths). Writes
+ * up to cap_rows rows (out may be NULL) and returns the number of rows. */
 int rcs_solver_rank_log(rcs_solver_t s, float* out, int cap_rows);
 /* ACA step launches, host checks and seconds spent in the assembly's ACA. */
 int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, double* seconds);

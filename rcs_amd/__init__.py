@@ -306,11 +306,12 @@ class AIFMM:
 
     def rank_log(self):
         """One row per ACA problem of the assembly: 8 predictor features, rank found,
-        rank predicted (0 without a model), level, 0 incoming / 1 outgoing."""
+        rank predicted (0 without a model), level, 0 incoming / 1 outgoing, short side
+        (box points), long side (interaction-list points)."""
         n = lib().rcs_solver_rank_log(self.h, None, 0)
         if n < 0:
             _check(n)
-        out = np.zeros((max(n, 1), 12), dtype=np.float32)
+        out = np.zeros((max(n, 1), 14), dtype=np.float32)
         lib().rcs_solver_rank_log(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
         return out[:n]
 

@@ -1,9 +1,18 @@
 // Batched ACA on gfx950: see aca.hpp. The per-step semantics follow src/ACA.cpp:639-859:
 // residual long vector at the current short index, first-max pivot over the remaining
 // long indices, v = residual / pivot, residual short vector at the pivot, u = that
-// vector, Frobenius estimate update (A:750-755), next short pivot = first max of |u|
-// over the remaining short indices, stop when |u||v| <= tol*|S| or rank = min(N1, N2).
+// vector, Frobenius estimate update (A:750-755, the sum over l includes the new pair),
+// next short pivot = first max of |u| over the remaining short indices, continue while
+// |u||v| > tol*|S| and rank < min(N1, N2).
+//
+// The stopping test of step k needs V(:,q)^H v_k for every q <= k, i.e. a pass over all of
+// V, and step k+1's residual long vector needs the same pass. Both are fused: step k+1's
+// entry kernel writes v_k = w_k / pivot into V, its long kernel forms the dots and the new
+// residual in one read of V, and its pivot kernel finishes step k's test before it pivots.
+// A box that stops has computed one residual long vector too many; its skeleton is
+// unchanged.
 #include <algorithm>
+#include <chrono>
 
 #include "aca.hpp"
 #include "entries_dev.hpp"
@@ -11,6 +20,8 @@
 namespace rcs {
 
 constexpr int ACA_CHUNK = 1024;  // long entries per workgroup (4 per thread)
+constexpr int ACA_SLOTS = 4;     // dot partials per chunk (one per wavefront)
+constexpr int ACA_PANEL = 64;    // V columns per separately allocated panel
 
 template <typename T>
 struct AcaBox {
@@ -19,19 +30,30 @@ struct AcaBox {
   int N1, N2, branch;  // branch 1: rows are the short side (N1 < N2)
   int ns, nl, cap, nchunks;
   T* U;  // ns x cap residual short vectors
-  T* V;  // nl x cap normalised long vectors
-  T* W;  // nl current long vector
+  // V, the normalised long vectors, in panels of ACA_PANEL columns, each stored chunk-major
+  // ([chunk][column][ACA_CHUNK]) so that one workgroup streams a contiguous block of it
+  T** Vp;
+  T* W;  // nl: residual long vector of the latest step
   unsigned char* remS;
   unsigned char* remL;
   int* pivS;
   int* pivL;
   double* pmax;  // nchunks
   int* pidx;     // nchunks
-  double* pdot;  // nchunks x (cap+1) x 2
-  int rank, active, init_done, s_cur, l_cur, retry;
+  double* pdot;  // (nchunks * ACA_SLOTS) x cap x 2: V(:,q)^H v of the pending step, per slot
+  double* du;    // cap x 2: U(:,q)^H u of the pending step
+  int rank;      // steps taken
+  int active, s_cur, l_cur, retry;
+  int pend;      // the latest step's v is not yet in V and its stopping test is not done
   double normS, uv;
-  T piv;
+  T piv;  // pivot of the latest step
 };
+
+// The ACA_CHUNK entries of V column q that belong to long chunk c.
+template <typename T>
+__device__ inline T* vcol(const AcaBox<T>& bx, int q, int c) {
+  return bx.Vp[q / ACA_PANEL] + ((size_t)c * ACA_PANEL + q % ACA_PANEL) * ACA_CHUNK;
+}
 
 template <typename R>
 __device__ inline R cabs_(cpx<R> a) { return sizeof(R) == 8 ? (R)hypot((double)a.re, (double)a.im) : (R)hypotf((float)a.re, (float)a.im); }
@@ -40,96 +62,15 @@ __device__ inline void argmax_combine(double& v, int& i, double v2, int i2) {
   if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
 }
 
-// K1: residual long vector + partial argmax over the remaining long indices.
-template <typename R, typename T>
-__global__ void __launch_bounds__(256) aca_long_kernel(DevGeometry g, int kind, AcaBox<T>* boxes, const int2* work) {
-  const int b = work[blockIdx.x].x, chunk = work[blockIdx.x].y;
-  AcaBox<T>& bx = boxes[b];
-  // a converged box, or one whose U/V capacity is full, takes no step (bursts of steps run
-  // between host checks, so the launch lists can be stale)
-  if (!bx.active || bx.rank >= bx.cap) return;
-  const int rank = bx.rank, s = bx.s_cur, nl = bx.nl;
-  __shared__ T us[256];
-  __shared__ double rv[256];
-  __shared__ int ri[256];
-  const int tid = threadIdx.x;
-  T val[4];
-  int tt[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    tt[e] = chunk * ACA_CHUNK + e * 256 + tid;
-    val[e] = T(0, 0);
-    if (tt[e] < nl) {
-      int i, j;
-      if (bx.branch == 1) { i = bx.rows[s]; j = bx.cols[tt[e]]; }
-      else { i = bx.rows[tt[e]]; j = bx.cols[s]; }
-      cpx<R> v = kernel_entry<R>(g, kind, i, j);
-      val[e] = T(v.re, v.im);
-    }
-  }
-  for (int q0 = 0; q0 < rank; q0 += 256) {
-    const int nq = min(256, rank - q0);
-    __syncthreads();
-    if (tid < nq) us[tid] = bx.U[(size_t)(q0 + tid) * bx.ns + s];
-    __syncthreads();
-    for (int q = 0; q < nq; ++q) {
-      const T uq = us[q];
-      const T* vq = bx.V + (size_t)(q0 + q) * nl;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (tt[e] < nl) val[e] = val[e] - uq * vq[tt[e]];
-    }
-  }
-  double best = -1.0;
-  int bi = 0x7fffffff;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (tt[e] < nl) {
-      bx.W[tt[e]] = val[e];
-      if (bx.remL[tt[e]]) {
-        double a = (double)cabs_(val[e]);
-        argmax_combine(best, bi, a, tt[e]);
-      }
-    }
-  }
-  rv[tid] = best;
-  ri[tid] = bi;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (tid < k) {
-      double v2 = rv[tid + k];
-      int i2 = ri[tid + k];
-      double v1 = rv[tid];
-      int i1 = ri[tid];
-      argmax_combine(v1, i1, v2, i2);
-      rv[tid] = v1;
-      ri[tid] = i1;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    bx.pmax[chunk] = rv[0];
-    bx.pidx[chunk] = ri[0];
+__device__ inline void wave_sum2(double& a, double& b) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
   }
 }
 
-// K2: pivot of the long vector (one workgroup per box).
-template <typename T>
-__global__ void __launch_bounds__(256) aca_pivot_kernel(AcaBox<T>* boxes, const int* ids) {
-  AcaBox<T>& bx = boxes[ids[blockIdx.x]];
-  if (!bx.active) return;
-  if (bx.rank >= bx.cap) {  // capacity full: skip the step (the host grows U/V, then resumes)
-    if (threadIdx.x == 0) bx.retry = 1;
-    return;
-  }
-  __shared__ double rv[256];
-  __shared__ int ri[256];
-  const int tid = threadIdx.x;
-  double best = -1.0;
-  int bi = 0x7fffffff;
-  for (int c = tid; c < bx.nchunks; c += 256) argmax_combine(best, bi, bx.pmax[c], bx.pidx[c]);
-  rv[tid] = best;
-  ri[tid] = bi;
+// First maximum over a workgroup of 256 (values in rv, indices in ri); result in rv[0], ri[0].
+__device__ inline void block_argmax(double* rv, int* ri, int tid) {
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if (tid < k) {
@@ -141,10 +82,211 @@ __global__ void __launch_bounds__(256) aca_pivot_kernel(AcaBox<T>* boxes, const 
     }
     __syncthreads();
   }
+}
+
+// K0 (one workgroup per chunk of a box's long side): the pending step's v = w / pivot into
+// V(:,rank-1), then this step's long entries (row or column of the box x interaction-list
+// block at short index s_cur) into W. Entry generation lives apart from the streaming
+// kernel below, whose occupancy the quadrature's registers would otherwise cap.
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) aca_long_entries_kernel(DevGeometry g, int kind, AcaBox<T>* boxes,
+                                                               const int2* work) {
+  const int b = work[blockIdx.x].x, chunk = work[blockIdx.x].y;
+  AcaBox<T>& bx = boxes[b];
+  // bursts of steps run between host checks, so the launch lists can be stale
+  if (!bx.active) return;
+  const int rank = bx.rank, nl = bx.nl, s = bx.s_cur;
+  const bool pend = bx.pend != 0, step = rank < bx.cap;
+  const T piv = bx.piv;
+  T* vc = pend ? vcol(bx, rank - 1, chunk) : nullptr;
+  for (int e = 0; e < ACA_CHUNK / 256; ++e) {
+    const int t = chunk * ACA_CHUNK + e * 256 + (int)threadIdx.x;
+    if (t >= nl) break;
+    if (pend) vc[e * 256 + threadIdx.x] = cdiv(bx.W[t], piv);
+    if (step) {
+      int i, j;
+      if (bx.branch == 1) { i = bx.rows[s]; j = bx.cols[t]; }
+      else { i = bx.rows[t]; j = bx.cols[s]; }
+      cpx<R> v = kernel_entry<R>(g, kind, i, j);
+      bx.W[t] = T(v.re, v.im);
+    }
+  }
+}
+
+// Sums x[0..7] over the 64 lanes of a wavefront with 10 exchanges instead of 48: at each of
+// the first three levels a lane keeps half of its values and trades the other half with its
+// partner. Lane l ends with the total of x[(l >> 3) & 7].
+__device__ inline double wave_sum8(const double (&x)[8], int lane) {
+  const int b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1;
+  double y[4], z[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b5 ? x[i + 4] : x[i], send = b5 ? x[i] : x[i + 4];
+    y[i] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b4 ? y[i + 2] : y[i], send = b4 ? y[i] : y[i + 2];
+    z[i] = keep + __shfl_xor(send, 16);
+  }
+  double w = (b3 ? z[1] : z[0]) + __shfl_xor(b3 ? z[0] : z[1], 8);
+  w += __shfl_xor(w, 4);
+  w += __shfl_xor(w, 2);
+  w += __shfl_xor(w, 1);
+  return w;
+}
+
+constexpr int ACA_QB = 4;  // V columns per iteration of the streaming loop (loads issued together)
+
+// K1 (one workgroup per chunk of a box's long side), one pass over V(:, 0..rank-1):
+//  pending step: partial dots V(:,q)^H v, q <= rank-1, with v = V(:,rank-1);
+//  this step (when U/V have room): residual long vector w = W - V U(s,:)^T -> W, partial
+//  first max of |w| over the remaining long indices.
+template <typename T>
+__global__ void __launch_bounds__(256) aca_long_kernel(AcaBox<T>* boxes, const int2* work) {
+  const int b = work[blockIdx.x].x, chunk = work[blockIdx.x].y;
+  AcaBox<T>& bx = boxes[b];
+  if (!bx.active) return;
+  const int rank = bx.rank, nl = bx.nl, cap = bx.cap, s = bx.s_cur;
+  const bool pend = bx.pend != 0;
+  const bool step = rank < cap;
+  if (!pend && !step) return;
+  __shared__ T us[ACA_PANEL];
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int tt[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) tt[e] = chunk * ACA_CHUNK + e * 256 + tid;
+  T vn[4], val[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const bool in = tt[e] < nl;
+    vn[e] = (pend && in) ? vcol(bx, rank - 1, chunk)[e * 256 + tid] : T(0, 0);
+    val[e] = (step && in) ? bx.W[tt[e]] : T(0, 0);
+  }
+  double* pd = bx.pdot + (size_t)(chunk * ACA_SLOTS + wave) * cap * 2;
+  for (int q0 = 0; q0 < rank; q0 += ACA_PANEL) {
+    const int nq = min(ACA_PANEL, rank - q0);
+    const T* vp = vcol(bx, q0, chunk);  // this panel's columns of the chunk, contiguous
+    if (step) {
+      __syncthreads();
+      if (tid < nq) us[tid] = bx.U[(size_t)(q0 + tid) * bx.ns + s];
+      __syncthreads();
+    }
+    for (int qq = 0; qq < nq; qq += ACA_QB) {
+      T a[ACA_QB][4];
+#pragma unroll
+      for (int i = 0; i < ACA_QB; ++i) {
+        const T* vq = vp + (size_t)(qq + i) * ACA_CHUNK;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[i][e] = (qq + i < nq && tt[e] < nl) ? vq[e * 256 + tid] : T(0, 0);
+      }
+      if (step) {
+#pragma unroll
+        for (int i = 0; i < ACA_QB; ++i) {
+          const T uq = qq + i < nq ? us[qq + i] : T(0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) val[e] = val[e] - uq * a[i][e];
+        }
+      }
+      if (pend) {
+        double d[2 * ACA_QB];
+#pragma unroll
+        for (int i = 0; i < ACA_QB; ++i) {
+          double sr = 0, si = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            sr += (double)a[i][e].re * vn[e].re + (double)a[i][e].im * vn[e].im;
+            si += (double)a[i][e].re * vn[e].im - (double)a[i][e].im * vn[e].re;
+          }
+          d[2 * i] = sr;
+          d[2 * i + 1] = si;
+        }
+        const double tot = wave_sum8(d, lane);
+        const int idx = lane >> 3;  // re/im of column qq + idx / 2
+        if ((lane & 7) == 0 && qq + (idx >> 1) < nq) pd[2 * (q0 + qq) + idx] = tot;
+      }
+    }
+  }
+  if (!step) return;
+  double best = -1.0;
+  int bi = 0x7fffffff;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (tt[e] < nl) {
+      bx.W[tt[e]] = val[e];
+      if (bx.remL[tt[e]]) argmax_combine(best, bi, (double)cabs_(val[e]), tt[e]);
+    }
+  }
+  rv[tid] = best;
+  ri[tid] = bi;
+  block_argmax(rv, ri, tid);
+  if (tid == 0) {
+    bx.pmax[chunk] = rv[0];
+    bx.pidx[chunk] = ri[0];
+  }
+}
+
+// K2 (one workgroup per box): the pending step's Frobenius update and stopping test, then
+// this step's long pivot.
+template <typename T>
+__global__ void __launch_bounds__(256) aca_pivot_kernel(AcaBox<T>* boxes, const int* ids, double tol) {
+  AcaBox<T>& bx = boxes[ids[blockIdx.x]];
+  if (!bx.active) return;
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ double nrm[2];
+  __shared__ int stop;
+  const int tid = threadIdx.x;
+  const int rank = bx.rank;
+  if (bx.pend) {
+    // |S|^2 += (|u||v|)^2 + sum_{q<=k} 2|U(:,q)^H u . V(:,q)^H v|  (A:750-755)
+    const int k = rank - 1, cap = bx.cap, nslots = bx.nchunks * ACA_SLOTS;
+    double acc = 0;
+    for (int q = tid; q <= k; q += 256) {
+      double dvr = 0, dvi = 0;
+      for (int sl = 0; sl < nslots; ++sl) {
+        const double* pd = bx.pdot + ((size_t)sl * cap + q) * 2;
+        dvr += pd[0];
+        dvi += pd[1];
+      }
+      const double dur = bx.du[2 * q], dui = bx.du[2 * q + 1];
+      const double pr = dur * dvr - dui * dvi, pi = dur * dvi + dui * dvr;
+      acc += 2.0 * sqrt(pr * pr + pi * pi);
+      if (q == k) { nrm[0] = dur; nrm[1] = dvr; }
+    }
+    rv[tid] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (tid < h) rv[tid] += rv[tid + h];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double uv = sqrt(nrm[0]) * sqrt(nrm[1]);
+      if (k > 0) bx.normS = sqrt(bx.normS * bx.normS + uv * uv + rv[0]);  // |S| is 0 after step 0
+      bx.uv = uv;
+      bx.pend = 0;
+      stop = (uv > tol * bx.normS) ? 0 : 1;
+      if (stop) bx.active = 0;
+    }
+    __syncthreads();
+    if (stop) return;
+  }
+  if (rank >= bx.cap) {  // U/V full: the host grows them, then the step runs
+    if (tid == 0) bx.retry = 1;
+    return;
+  }
+  double best = -1.0;
+  int bi = 0x7fffffff;
+  for (int c = tid; c < bx.nchunks; c += 256) argmax_combine(best, bi, bx.pmax[c], bx.pidx[c]);
+  rv[tid] = best;
+  ri[tid] = bi;
+  block_argmax(rv, ri, tid);
   if (tid == 0) {
     const int l = ri[0];
     const double pv = rv[0];
-    if (!bx.init_done && !(pv > 1e-16)) {  // A:663-669: skip an all-zero first vector
+    if (rank == 0 && !(pv > 1e-16)) {  // A:663-669: skip an all-zero first vector
       bx.retry = 1;
       bx.s_cur += 1;
       if (bx.s_cur >= bx.ns) bx.active = 0;
@@ -158,165 +300,81 @@ __global__ void __launch_bounds__(256) aca_pivot_kernel(AcaBox<T>* boxes, const 
     bx.retry = 0;
     bx.l_cur = l;
     bx.piv = bx.W[l];
-    bx.pivS[bx.rank] = bx.s_cur;
-    bx.pivL[bx.rank] = l;
+    bx.pivS[rank] = bx.s_cur;
+    bx.pivL[rank] = l;
     bx.remL[l] = 0;
     bx.remS[bx.s_cur] = 0;
   }
 }
 
-// K3: v = w / pivot, stored as column `rank` of V, with partial dots V(:,q)^H v.
-template <typename T>
-__global__ void __launch_bounds__(256) aca_norm_kernel(AcaBox<T>* boxes, const int2* work) {
-  const int b = work[blockIdx.x].x, chunk = work[blockIdx.x].y;
+// K2b (one workgroup per 256 short indices of a box): this step's short entries (column or
+// row of the block at the long pivot) into U(:,rank).
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) aca_short_entries_kernel(DevGeometry g, int kind, AcaBox<T>* boxes,
+                                                                const int2* work) {
+  const int b = work[blockIdx.x].x, t = work[blockIdx.x].y * 256 + (int)threadIdx.x;
   AcaBox<T>& bx = boxes[b];
-  if (bx.retry || !bx.active) return;
-  const int rank = bx.rank, nl = bx.nl;
-  const T piv = bx.piv;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ double wr[4], wi[4];
-  T v[4];
-  int tt[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    tt[e] = chunk * ACA_CHUNK + e * 256 + tid;
-    v[e] = T(0, 0);
-    if (tt[e] < nl) {
-      v[e] = cdiv(bx.W[tt[e]], piv);
-      bx.V[(size_t)rank * nl + tt[e]] = v[e];
-    }
-  }
-  for (int q = 0; q <= rank; ++q) {
-    double sr = 0, si = 0;
-    if (q < rank) {
-      const T* vq = bx.V + (size_t)q * nl;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (tt[e] < nl) {
-          T a = vq[tt[e]];
-          sr += (double)a.re * v[e].re + (double)a.im * v[e].im;
-          si += (double)a.re * v[e].im - (double)a.im * v[e].re;
-        }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (tt[e] < nl) sr += (double)v[e].re * v[e].re + (double)v[e].im * v[e].im;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      sr += __shfl_xor(sr, off);
-      si += __shfl_xor(si, off);
-    }
-    if (lane == 0) { wr[wave] = sr; wi[wave] = si; }
-    __syncthreads();
-    if (tid == 0) {
-      double* pd = bx.pdot + ((size_t)chunk * (bx.cap + 1) + q) * 2;
-      pd[0] = (wr[0] + wr[1]) + (wr[2] + wr[3]);
-      pd[1] = (wi[0] + wi[1]) + (wi[2] + wi[3]);
-    }
-    __syncthreads();
-  }
+  if (bx.retry || !bx.active || t >= bx.ns) return;
+  const int l = bx.l_cur;
+  int i, j;
+  if (bx.branch == 1) { i = bx.rows[t]; j = bx.cols[l]; }
+  else { i = bx.rows[l]; j = bx.cols[t]; }
+  cpx<R> e = kernel_entry<R>(g, kind, i, j);
+  bx.U[(size_t)bx.rank * bx.ns + t] = T(e.re, e.im);
 }
 
-// K4: residual short vector, dots, Frobenius update, stopping test, next short pivot.
-template <typename R, typename T>
-__global__ void __launch_bounds__(256) aca_short_kernel(DevGeometry g, int kind, AcaBox<T>* boxes, const int* ids,
-                                                        double tol) {
+// K3 (one workgroup per box): residual short vector u = U(:,rank) - U V(l,:)^T at the long
+// pivot, its dots U(:,q)^H u for q <= rank (one wavefront per column), next short pivot.
+template <typename T>
+__global__ void __launch_bounds__(256) aca_short_kernel(AcaBox<T>* boxes, const int* ids) {
   AcaBox<T>& bx = boxes[ids[blockIdx.x]];
   if (bx.retry || !bx.active) return;
-  const int rank = bx.rank, ns = bx.ns, l = bx.l_cur;
+  const int rank = bx.rank, ns = bx.ns, nl = bx.nl, l = bx.l_cur;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ T vs[256];
-  __shared__ double wr[4], wi[4];
-  __shared__ double sdu_r[1], sdu_i[1];
+  __shared__ T vs[ACA_PANEL];
   __shared__ double rv[256];
   __shared__ int ri[256];
   T* u = bx.U + (size_t)rank * ns;
-  // residual short vector (kept in HBM: ns can exceed the workgroup)
-  for (int t = tid; t < ns; t += 256) {
-    int i, j;
-    if (bx.branch == 1) { i = bx.rows[t]; j = bx.cols[l]; }
-    else { i = bx.rows[l]; j = bx.cols[t]; }
-    cpx<R> e = kernel_entry<R>(g, kind, i, j);
-    u[t] = T(e.re, e.im);
-  }
-  __syncthreads();
-  for (int q0 = 0; q0 < rank; q0 += 256) {
-    const int nq = min(256, rank - q0);
-    if (tid < nq) vs[tid] = bx.V[(size_t)(q0 + tid) * bx.nl + l];
+  for (int q0 = 0; q0 < rank; q0 += ACA_PANEL) {
+    const int nq = min(ACA_PANEL, rank - q0);
+    __syncthreads();
+    if (tid < nq) vs[tid] = vcol(bx, q0 + tid, l / ACA_CHUNK)[l % ACA_CHUNK];
     __syncthreads();
     for (int t = tid; t < ns; t += 256) {
       T a = u[t];
       for (int q = 0; q < nq; ++q) a = a - vs[q] * bx.U[(size_t)(q0 + q) * ns + t];
       u[t] = a;
     }
-    __syncthreads();
   }
-  // dots U(:,q)^H u, q <= rank (q = rank gives |u|^2), combined with the V dots
-  double acc = 0;  // sum_q 2|dU_q dV_q|
-  double un2 = 0, vn2 = 0;
-  for (int q = 0; q <= rank; ++q) {
-    double sr = 0, si = 0;
+  __syncthreads();
+  for (int q = wave; q <= rank; q += 4) {
     const T* uq = bx.U + (size_t)q * ns;
-    for (int t = tid; t < ns; t += 256) {
-      T a = uq[t], c = u[t];
+    double sr = 0, si = 0;
+    for (int t = lane; t < ns; t += 64) {
+      const T a = uq[t], c = u[t];
       sr += (double)a.re * c.re + (double)a.im * c.im;
       si += (double)a.re * c.im - (double)a.im * c.re;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      sr += __shfl_xor(sr, off);
-      si += __shfl_xor(si, off);
+    wave_sum2(sr, si);
+    if (lane == 0) {
+      bx.du[2 * q] = sr;
+      bx.du[2 * q + 1] = si;
     }
-    if (lane == 0) { wr[wave] = sr; wi[wave] = si; }
-    __syncthreads();
-    if (tid == 0) {
-      double dur = (wr[0] + wr[1]) + (wr[2] + wr[3]);
-      double dui = (wi[0] + wi[1]) + (wi[2] + wi[3]);
-      double dvr = 0, dvi = 0;
-      for (int c = 0; c < bx.nchunks; ++c) {
-        const double* pd = bx.pdot + ((size_t)c * (bx.cap + 1) + q) * 2;
-        dvr += pd[0];
-        dvi += pd[1];
-      }
-      if (q == rank) { un2 = dur; vn2 = dvr; }
-      double pr = dur * dvr - dui * dvi, pi = dur * dvi + dui * dvr;
-      acc += 2.0 * sqrt(pr * pr + pi * pi);
-      sdu_r[0] = dur;
-      sdu_i[0] = dui;
-    }
-    __syncthreads();
   }
-  // next short pivot: first max |u| over the remaining short indices
   double best = -1.0;
   int bi = 0x7fffffff;
   for (int t = tid; t < ns; t += 256)
     if (bx.remS[t]) argmax_combine(best, bi, (double)cabs_(u[t]), t);
   rv[tid] = best;
   ri[tid] = bi;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (tid < k) {
-      double v1 = rv[tid];
-      int i1 = ri[tid];
-      argmax_combine(v1, i1, rv[tid + k], ri[tid + k]);
-      rv[tid] = v1;
-      ri[tid] = i1;
-    }
-    __syncthreads();
-  }
+  block_argmax(rv, ri, tid);
   if (tid == 0) {
-    const double uv = sqrt(un2) * sqrt(vn2);
-    if (bx.init_done) {
-      double ns2 = bx.normS * bx.normS + uv * uv + acc;
-      bx.normS = sqrt(ns2);
-    }
-    bx.init_done = 1;
-    bx.uv = uv;
-    bx.rank = rank + 1;
-    const int mn = min(bx.N1, bx.N2);
-    const int snext = ri[0];
-    const bool more = (uv > tol * bx.normS) && (bx.rank < mn) && (snext >= 0 && snext < ns);
-    bx.active = more ? 1 : 0;
-    if (more) bx.s_cur = snext;
+    const int nr = rank + 1, snext = ri[0];
+    bx.rank = nr;
+    bx.pend = 1;
+    // rank = min(N1, N2) ends the loop without the Frobenius test (A:709, A:802)
+    if (nr >= min(bx.N1, bx.N2) || snext < 0 || snext >= ns) bx.active = 0;
+    else bx.s_cur = snext;
   }
 }
 
@@ -329,13 +387,24 @@ template <typename T>
 static void aca_group(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
                       std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats);
 
-// The residual vectors of every box of a group stay in HBM until the group converges
-// (long side x rank each), so the boxes of a level are processed in groups whose expected
-// footprint stays under a budget.
+// U/V capacity in whole V panels: the predicted rank with a margin when there is one.
+static int aca_cap0(int ns, int pred) {
+  const int want = pred > 0 ? pred + pred / 4 + 8 : ACA_PANEL;
+  return std::min(ns, (want + ACA_PANEL - 1) / ACA_PANEL * ACA_PANEL);
+}
+
+// The vectors of every box of a group stay in HBM until the group converges (long + short
+// side x rank each), so the boxes of a level run in groups whose expected footprint (at a
+// rank of 256, or the predicted capacity) fits in half of the device memory that is free
+// or cached by the pool. The pool keeps what the ACA frees: V panels have one size per
+// long side, so the next direction and level reuse them without going back to the driver.
 template <typename T>
 void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<AcaProblem>& probs,
                  std::vector<AcaResult>& out, Ctx& cx, AcaStats& stats) {
-  const size_t budget = size_t(24) << 30;
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  const size_t avail = fr + (cx.pool.total() - cx.pool.in_use());
+  const size_t budget = std::max<size_t>(avail / 2, size_t(1) << 30);
   out.assign(probs.size(), AcaResult());
   std::vector<size_t> grp;
   size_t used = 0;
@@ -352,7 +421,7 @@ void aca_batched(const DevGeometry& g, int kind, double tol, const std::vector<A
   for (size_t b = 0; b < probs.size(); ++b) {
     const size_t n1 = probs[b].rows.size(), n2 = probs[b].cols.size();
     const size_t ns = std::min(n1, n2), nl = std::max(n1, n2);
-    const size_t cap = std::min<size_t>(ns, probs[b].pred > 0 ? probs[b].pred + probs[b].pred / 4 + 8 : 256);
+    const size_t cap = probs[b].pred > 0 ? (size_t)aca_cap0((int)ns, probs[b].pred) : std::min<size_t>(ns, 256);
     const size_t bytes = (nl + ns) * cap * sizeof(T);
     if (!grp.empty() && used + bytes > budget) flush();
     grp.push_back(b);
@@ -369,11 +438,21 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
   out.assign(nb, AcaResult());
   if (nb == 0) return;
   std::vector<AcaBox<T>> hb(nb);
+  std::vector<std::vector<T*>> panels(nb);  // V panels of each box (host copy of its table)
   std::vector<void*> owned;
   auto alloc = [&](size_t bytes) {
     void* p = cx.pool.alloc(bytes);
     owned.push_back(p);
     return p;
+  };
+  auto pdot_bytes = [](const AcaBox<T>& x) { return sizeof(double) * 2 * (size_t)x.nchunks * ACA_SLOTS * x.cap; };
+  // V panels for columns [0, cap): allocate the missing ones and publish their pointers
+  auto add_panels = [&](int b) {
+    AcaBox<T>& x = hb[b];
+    const int have = (int)panels[b].size(), want = (x.cap + ACA_PANEL - 1) / ACA_PANEL;
+    for (int p = have; p < want; ++p)
+      panels[b].push_back((T*)cx.pool.alloc(sizeof(T) * (size_t)x.nchunks * ACA_CHUNK * ACA_PANEL));
+    if (want > have) cx.upload(x.Vp + have, panels[b].data() + have, sizeof(T*) * (want - have));
   };
   std::vector<int> live;
   for (int b = 0; b < nb; ++b) {
@@ -386,8 +465,7 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
     x.branch = (x.N1 < x.N2) ? 1 : 2;
     x.ns = std::min(x.N1, x.N2);
     x.nl = std::max(x.N1, x.N2);
-    // U/V capacity: the predicted rank with a margin when there is one
-    x.cap = P.pred > 0 ? std::min(x.ns, P.pred + P.pred / 4 + 8) : std::min(x.ns, 64);
+    x.cap = aca_cap0(x.ns, P.pred);
     x.nchunks = (x.nl + ACA_CHUNK - 1) / ACA_CHUNK;
     int* dr = (int*)alloc(sizeof(int) * x.N1);
     int* dc = (int*)alloc(sizeof(int) * x.N2);
@@ -395,8 +473,9 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
     cx.upload(dc, P.cols.data(), sizeof(int) * x.N2);
     x.rows = dr;
     x.cols = dc;
+    x.Vp = (T**)alloc(sizeof(T*) * ((x.ns + ACA_PANEL - 1) / ACA_PANEL));
+    add_panels(b);
     x.U = (T*)cx.pool.alloc(sizeof(T) * (size_t)x.ns * x.cap);
-    x.V = (T*)cx.pool.alloc(sizeof(T) * (size_t)x.nl * x.cap);
     x.W = (T*)alloc(sizeof(T) * (size_t)x.nl);
     x.remS = (unsigned char*)alloc(x.ns);
     x.remL = (unsigned char*)alloc(x.nl);
@@ -404,7 +483,8 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
     x.pivL = (int*)alloc(sizeof(int) * x.ns);
     x.pmax = (double*)alloc(sizeof(double) * x.nchunks);
     x.pidx = (int*)alloc(sizeof(int) * x.nchunks);
-    x.pdot = (double*)cx.pool.alloc(sizeof(double) * 2 * (size_t)x.nchunks * (x.cap + 1));
+    x.pdot = (double*)cx.pool.alloc(pdot_bytes(x));
+    x.du = (double*)cx.pool.alloc(sizeof(double) * 2 * (size_t)x.cap);
     hipLaunchKernelGGL(fill_u8_kernel, dim3((x.ns + 255) / 256), dim3(256), 0, cx.st, x.remS, (size_t)x.ns, (unsigned char)1);
     hipLaunchKernelGGL(fill_u8_kernel, dim3((x.nl + 255) / 256), dim3(256), 0, cx.st, x.remL, (size_t)x.nl, (unsigned char)1);
     x.active = 1;
@@ -418,59 +498,76 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
   for (int b : live) predicted = predicted || probs[b].pred > 0;
   bool first = true;
   while (!active.empty()) {
-    std::vector<int2> work;
-    for (int b : active)
+    std::vector<int2> work, swork;
+    for (int b : active) {
       for (int c = 0; c < hb[b].nchunks; ++c) work.push_back(make_int2(b, c));
+      for (int c = 0; c < (hb[b].ns + 255) / 256; ++c) swork.push_back(make_int2(b, c));
+    }
     // steps between host checks: the predicted remaining rank on the first pass, then the
-    // fixed burst (converged boxes skip the extra steps inside the kernels, so a burst
-    // never changes the skeletons, only how often the host looks)
+    // fixed burst (stopped boxes skip the extra steps inside the kernels, so a burst never
+    // changes the skeletons, only how often the host looks)
     int burst = std::max(1, stats.burst);
-    if (predicted && first) {  // the prediction plus an eighth: extra steps of a converged box are no-ops
+    if (predicted && first) {
       int need = 1;
       for (int b : active) {
         const int p = probs[b].pred + probs[b].pred / 8 + 2;
         need = std::max(need, std::min(p, hb[b].cap) - hb[b].rank);
       }
-      burst = std::min(need, 256);
+      burst = std::min(need + 1, 257);  // + 1: the last step's test runs in the next step's launches
     }
     first = false;
     const int2* d_work = (const int2*)cx.push(work.data(), sizeof(int2) * work.size());
+    const int2* d_swork = (const int2*)cx.push(swork.data(), sizeof(int2) * swork.size());
     const int* d_ids = (const int*)cx.push(active.data(), sizeof(int) * active.size());
     for (int st = 0; st < burst; ++st) {
-      hipLaunchKernelGGL((aca_long_kernel<R, T>), dim3(work.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_work);
-      hipLaunchKernelGGL(aca_pivot_kernel<T>, dim3(active.size()), dim3(256), 0, cx.st, d_boxes, d_ids);
-      hipLaunchKernelGGL(aca_norm_kernel<T>, dim3(work.size()), dim3(256), 0, cx.st, d_boxes, d_work);
-      hipLaunchKernelGGL((aca_short_kernel<R, T>), dim3(active.size()), dim3(256), 0, cx.st, g, kind, d_boxes, d_ids,
-                         tol);
+      hipLaunchKernelGGL((aca_long_entries_kernel<R, T>), dim3(work.size()), dim3(256), 0, cx.st, g, kind, d_boxes,
+                         d_work);
+      hipLaunchKernelGGL(aca_long_kernel<T>, dim3(work.size()), dim3(256), 0, cx.st, d_boxes, d_work);
+      hipLaunchKernelGGL(aca_pivot_kernel<T>, dim3(active.size()), dim3(256), 0, cx.st, d_boxes, d_ids, tol);
+      hipLaunchKernelGGL((aca_short_entries_kernel<R, T>), dim3(swork.size()), dim3(256), 0, cx.st, g, kind, d_boxes,
+                         d_swork);
+      hipLaunchKernelGGL(aca_short_kernel<T>, dim3(active.size()), dim3(256), 0, cx.st, d_boxes, d_ids);
     }
     stats.steps += burst;
     stats.syncs += 1;
     HIPCHK(hipGetLastError());
+    const auto w0 = std::chrono::steady_clock::now();
     HIPCHK(hipMemcpyAsync(hb.data(), d_boxes, sizeof(AcaBox<T>) * nb, hipMemcpyDeviceToHost, cx.st));
     cx.sync();
+    const auto w1 = std::chrono::steady_clock::now();
+    stats.wait_s += std::chrono::duration<double>(w1 - w0).count();
+    // grow U/V ahead of the next burst, so that a box rarely waits at a full capacity: more
+    // V panels (half as many again), U and the pending U dots copied into larger blocks, new
+    // dot partials. Every launch is on the one stream, so blocks given back to the pool
+    // here are not reused before the copies out of them have run.
+    const int margin = std::max(1, stats.burst) + 1;
     std::vector<int> next;
     bool grown = false;
     for (int b : active) {
       AcaBox<T>& x = hb[b];
       if (!x.active) continue;
       next.push_back(b);
-      if (x.rank >= x.cap) {  // grow U, V and the dot partials
-        int ncap = std::min(x.ns, x.cap * 2);
+      if (x.cap < x.ns && x.rank + margin > x.cap) {
+        const int np = (x.cap + ACA_PANEL - 1) / ACA_PANEL;
+        const int ncap = std::min(x.ns, (np + std::max(1, np / 2)) * ACA_PANEL);
         T* nU = (T*)cx.pool.alloc(sizeof(T) * (size_t)x.ns * ncap);
-        T* nV = (T*)cx.pool.alloc(sizeof(T) * (size_t)x.nl * ncap);
+        double* ndu = (double*)cx.pool.alloc(sizeof(double) * 2 * (size_t)ncap);
         HIPCHK(hipMemcpyAsync(nU, x.U, sizeof(T) * (size_t)x.ns * x.cap, hipMemcpyDeviceToDevice, cx.st));
-        HIPCHK(hipMemcpyAsync(nV, x.V, sizeof(T) * (size_t)x.nl * x.cap, hipMemcpyDeviceToDevice, cx.st));
+        HIPCHK(hipMemcpyAsync(ndu, x.du, sizeof(double) * 2 * (size_t)x.cap, hipMemcpyDeviceToDevice, cx.st));
         cx.pool.free(x.U);
-        cx.pool.free(x.V);
+        cx.pool.free(x.du);
         cx.pool.free(x.pdot);
         x.U = nU;
-        x.V = nV;
+        x.du = ndu;
         x.cap = ncap;
-        x.pdot = (double*)cx.pool.alloc(sizeof(double) * 2 * (size_t)x.nchunks * (x.cap + 1));
+        x.pdot = (double*)cx.pool.alloc(pdot_bytes(x));
+        add_panels(b);
+        x.retry = 0;
         grown = true;
       }
     }
     if (grown) cx.upload(d_boxes, hb.data(), sizeof(AcaBox<T>) * nb);
+    stats.grow_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - w1).count();
     active.swap(next);
   }
   for (int b : live) {
@@ -485,9 +582,10 @@ static void aca_group(const DevGeometry& g, int kind, double tol, const std::vec
     cx.sync();
     if (x.branch == 1) { r.row_bases = ps; r.col_bases = pl; }
     else { r.row_bases = pl; r.col_bases = ps; }
+    for (T* p : panels[b]) cx.pool.free(p);
     cx.pool.free(x.U);
-    cx.pool.free(x.V);
     cx.pool.free(x.pdot);
+    cx.pool.free(x.du);
   }
   for (void* p : owned) cx.pool.free(p);
   cx.pool.free(d_boxes);

@@ -1,11 +1,19 @@
 // Batched adaptive cross approximation (LowRank<K>::ACA_only_nodes2, src/ACA.cpp:639-859).
 //
 // All boxes of one tree level (one direction: incoming or outgoing) run the pivoted
-// cross approximation together: every step is one launch that generates the residual
-// "long" vector of every active box (row or column of the box x interaction-list
-// block, entries computed in the kernel), a pivot launch (wavefront argmax), a
-// normalisation launch and one launch for the short vectors, stopping test and next
-// pivot. Per-box state stays in HBM; the host only reads the active flags and ranks.
+// cross approximation together, five launches per step:
+//  - long entries: one workgroup per 1024-entry chunk of every active box's long side. It
+//    normalises the previous step's long vector into V and generates this step's long
+//    entries (the matrix entries are computed in the kernel).
+//  - long: same grid. It takes the previous step's dots with every earlier V column (for
+//    the Frobenius estimate) and forms this step's residual long vector from the same
+//    pass over V.
+//  - pivot: one workgroup per box. It finishes the previous step's Frobenius estimate,
+//    applies the stopping test, then picks the long pivot (first maximum).
+//  - short entries: one workgroup per 256 short indices.
+//  - short: one workgroup per box. It forms the residual short vector and its dots with
+//    the earlier U columns, and picks the next short pivot.
+// Per-box state stays in HBM; the host only reads the active flags and ranks.
 #pragma once
 #include "ctx.hpp"
 #include "dense.hpp"
@@ -25,6 +33,8 @@ struct AcaStats {
   int burst = 1;            // in: ACA steps launched between host checks (after the first pass)
   long long steps = 0;      // out: step launches
   long long syncs = 0;      // out: host synchronisations
+  double grow_s = 0;        // out: seconds spent growing U/V (allocation and copies)
+  double wait_s = 0;        // out: seconds the host waited for step bursts
 };
 
 // Runs ACA on every problem (tolerance tol) and returns the skeletons. The skeletons do not

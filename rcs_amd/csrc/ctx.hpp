@@ -28,6 +28,13 @@ class DevPool {
       in_use_ += sz;
       return p;
     }
+    // blocks cached here are HBM the HIP runtime cannot use: past the limit they go back first
+    if (limit_ == 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      limit_ = tot / 100 * 85;
+    }
+    if (total_ + bytes > limit_) trim();
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -67,6 +74,7 @@ class DevPool {
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> used_;
   size_t total_ = 0, in_use_ = 0, peak_ = 0;
+  size_t limit_ = 0;  // 85% of the device memory, set at the first allocation
   bool no_reuse_ = std::getenv("RCS_DEBUG_NO_REUSE") != nullptr;  // debugging: never recycle blocks
 };
 

@@ -83,13 +83,13 @@ class Solver : public SolverBase {
   using R = decltype(T().re);
   Solver(std::shared_ptr<DeviceGeometry> g, const SolverParams& p, Ctx& c) : G(g), P(p), cx(c) {}
   // Box blocks are plain views; the solver owns every pool block it allocated and hands
-  // all of them back (and the pool's cached blocks to the driver) when it goes away, so a
-  // process can build solvers one after another without holding the earlier ones' HBM.
+  // all of them back to the pool when it goes away, so a process can build solvers one
+  // after another without holding the earlier ones' HBM (the next solver reuses the cached
+  // blocks; the pool returns them to the driver when it nears its limit).
   ~Solver() override {
     cx.sync();
     for (auto& kv : owned_) cx.pool.free(kv.first);
     owned_.clear();
-    cx.pool.trim();
   }
 
   // ------------------------------------------------------------ block helpers
@@ -406,12 +406,18 @@ class Solver : public SolverBase {
     stats_.t_aca += wall() - t0;
     stats_.aca_steps += ast.steps;
     stats_.aca_syncs += ast.syncs;
+    if (verbose)
+      std::fprintf(stderr, "[rcs] aca level %d dir %d: %lld steps, %.3f s, waiting %.3f s, growing %.3f s\n", j, inout,
+                   ast.steps, wall() - t0, ast.wait_s, ast.grow_s);
     for (size_t i = 0; i < which.size(); ++i) {
       rank_log_.insert(rank_log_.end(), feat.begin() + i * MLP_IN, feat.begin() + (i + 1) * MLP_IN);
       rank_log_.push_back((float)res[which[i]].rank);
       rank_log_.push_back((float)pred[i]);
       rank_log_.push_back((float)j);
       rank_log_.push_back((float)inout);
+      const AcaProblem& p = probs[which[i]];
+      rank_log_.push_back((float)std::min(p.rows.size(), p.cols.size()));
+      rank_log_.push_back((float)std::max(p.rows.size(), p.cols.size()));
     }
   }
   const std::vector<float>& rank_log() const override { return rank_log_; }
@@ -620,10 +626,6 @@ class Solver : public SolverBase {
     setup();
     trace("setup: N=%d nLevels=%d", N, nLevels);
     getNodes();
-    // the ACA residual vectors were transient: hand their cached blocks back to the device
-    // (the HIP runtime's own allocations, e.g. kernel scratch, come from the same HBM)
-    cx.sync();
-    cx.pool.trim();
     trace("skeletons done, startLevel=%d", startLevel);
     assemble_M2L();
     free_points();

@@ -44,7 +44,7 @@ struct SolverStats {
 // One ACA problem of the skeleton construction: the predictor's 8 features, the rank the
 // adaptive ACA found, the rank predicted (0 without a model), the level, 0 incoming / 1
 // outgoing.
-constexpr int RANK_LOG_COLS = 12;
+constexpr int RANK_LOG_COLS = 14;
 
 class DeviceGeometry {
  public:
