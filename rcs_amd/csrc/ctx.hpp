@@ -38,6 +38,7 @@ class DevPool {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
+      (void)hipGetLastError();  // the failure is handled here: do not leave it for the next check
       trim();
       HIPCHK(hipMalloc(&p, bytes));
     }

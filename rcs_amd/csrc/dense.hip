@@ -50,59 +50,33 @@ struct GemmArgs {
   int nouts;
 };
 
+// Products of an output block over the global inner-index range [klo, khi) (the inner
+// dimensions of its segments laid end to end) into acc.
 template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmOut<T>* outs, int nouts, const GemmSeg<T>* segs);
-
-// ROLE only separates launch sites in kernel traces (1: Schur-complement updates of the
-// elimination); the code is the same.
-template <typename T, int ROLE>
-__global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOut<T>* outs, int nouts,
-                                                           const GemmSeg<T>* segs) {
-  gemm_tile(outs, nouts, segs);
-}
-template <typename T, int ROLE>
-__global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs<T> a) {
-  gemm_tile(a.outs, a.nouts, a.segs);
-}
-
-template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmOut<T>* outs, int nouts, const GemmSeg<T>* segs) {
-  using R = typename RealOf<T>::type;
+__device__ __forceinline__ void gemm_acc(const GemmOut<T>& out, const GemmSeg<T>* segs, int m0, int n0, int klo,
+                                         int khi, T (&acc)[4][4]) {
   __shared__ T As[GK][GT + 1];
   __shared__ T Bs[GK][GT + 1];
-  const int tile = blockIdx.x;
-  int lo = 0, hi = nouts - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (outs[mid].tile0 <= tile) lo = mid;
-    else hi = mid - 1;
-  }
-  const GemmOut<T> out = outs[lo];
-  const int t = tile - out.tile0;
-  const int tm = (out.m + GT - 1) / GT;
-  const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
+  int kbase = 0;
   for (int sgi = 0; sgi < out.nseg; ++sgi) {
     const GemmSeg<T> op = segs[out.seg0 + sgi];
-    for (int k0 = 0; k0 < op.k; k0 += GK) {
+    const int lo = max(klo - kbase, 0), hi = min(khi - kbase, op.k);
+    kbase += op.k;
+    for (int k0 = lo; k0 < hi; k0 += GK) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         int i, kk;
         if (op.opA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * s; }
         else { kk = tid & 15; i = (tid >> 4) + 16 * s; }
         T v(0, 0);
-        if (m0 + i < out.m && k0 + kk < op.k) v = ld_op(op.A, op.lda, m0 + i, k0 + kk, op.opA);
+        if (m0 + i < out.m && k0 + kk < hi) v = ld_op(op.A, op.lda, m0 + i, k0 + kk, op.opA);
         As[kk][i] = v;
         int j2, k2;
         if (op.opB == OP_N) { k2 = tid & 15; j2 = (tid >> 4) + 16 * s; }
         else { j2 = tid & 63; k2 = (tid >> 6) + 4 * s; }
         T w(0, 0);
-        if (n0 + j2 < out.n && k0 + k2 < op.k) w = ld_op(op.B, op.ldb, k0 + k2, n0 + j2, op.opB);
+        if (n0 + j2 < out.n && k0 + k2 < hi) w = ld_op(op.B, op.ldb, k0 + k2, n0 + j2, op.opB);
         Bs[k2][j2] = w;
       }
       __syncthreads();
@@ -121,6 +95,24 @@ __device__ __forceinline__ void gemm_tile(const GemmOut<T>* outs, int nouts, con
       __syncthreads();
     }
   }
+}
+
+template <typename T>
+__device__ __forceinline__ int find_out(const GemmOut<T>* outs, int nouts, int tile) {
+  int lo = 0, hi = nouts - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (outs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// C = alpha * acc + beta * C for the tile at (m0, n0).
+template <typename T>
+__device__ __forceinline__ void gemm_store(const GemmOut<T>& out, int m0, int n0, const T (&acc)[4][4]) {
+  using R = typename RealOf<T>::type;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const bool zb = (out.beta.re == R(0) && out.beta.im == R(0));
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -137,6 +129,84 @@ __device__ __forceinline__ void gemm_tile(const GemmOut<T>* outs, int nouts, con
     }
   }
 }
+
+template <typename T>
+__device__ __forceinline__ void gemm_tile(const GemmOut<T>* outs, int nouts, const GemmSeg<T>* segs) {
+  const int tile = blockIdx.x;
+  const GemmOut<T> out = outs[find_out(outs, nouts, tile)];
+  const int t = tile - out.tile0;
+  const int tm = (out.m + GT - 1) / GT;
+  const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
+  gemm_acc(out, segs, m0, n0, 0, 0x7fffffff, acc);
+  gemm_store(out, m0, n0, acc);
+}
+
+// ROLE only separates launch sites in kernel traces (1: Schur-complement updates of the
+// elimination); the code is the same.
+template <typename T, int ROLE>
+__global__ void __launch_bounds__(256) gemm_grouped_kernel(const GemmOut<T>* outs, int nouts,
+                                                           const GemmSeg<T>* segs) {
+  gemm_tile(outs, nouts, segs);
+}
+template <typename T, int ROLE>
+__global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs<T> a) {
+  gemm_tile(a.outs, a.nouts, a.segs);
+}
+
+// Split inner dimension for groups with few output tiles: workgroup (tile, part) sums the
+// products over its slice of the inner index into part's 64x64 slot of `partial`; the
+// reduce kernel adds the parts of a tile in a fixed order and applies alpha and beta.
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_splitk_kernel(GemmArgs<T> a, int parts, T* partial) {
+  const int tile = blockIdx.x / parts, part = blockIdx.x % parts;
+  const GemmOut<T> out = a.outs[find_out(a.outs, a.nouts, tile)];
+  int ktot = 0;
+  for (int sgi = 0; sgi < out.nseg; ++sgi) ktot += a.segs[out.seg0 + sgi].k;
+  const int kc = ((ktot + parts - 1) / parts + GK - 1) / GK * GK;
+  const int t = tile - out.tile0, tm = (out.m + GT - 1) / GT;
+  const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
+  gemm_acc(out, a.segs, m0, n0, part * kc, min((part + 1) * kc, ktot), acc);
+  T* slot = partial + (size_t)blockIdx.x * GT * GT;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) slot[(ty + 16 * i) + (tx + 16 * j) * GT] = acc[i][j];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_splitk_reduce_kernel(GemmArgs<T> a, int parts, const T* partial) {
+  const int tile = blockIdx.x;
+  const GemmOut<T> out = a.outs[find_out(a.outs, a.nouts, tile)];
+  const int t = tile - out.tile0, tm = (out.m + GT - 1) / GT;
+  const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
+  for (int p = 0; p < parts; ++p) {
+    const T* slot = partial + ((size_t)tile * parts + p) * GT * GT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + slot[(ty + 16 * i) + (tx + 16 * j) * GT];
+  }
+  gemm_store(out, m0, n0, acc);
+}
+
+constexpr int SPLITK_TILES = 96;    // groups with fewer output tiles than this split the inner index
+constexpr int SPLITK_MIN_K = 64;    // inner index per part, at least
 
 template <typename T>
 void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) {
@@ -180,17 +250,32 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
   GemmArgs<T> args;
   const GemmOut<T>* d_outs = nullptr;
   const GemmSeg<T>* d_segs = nullptr;
+  int parts = 1;
   if (small) {
     std::copy(outs.begin(), outs.end(), args.outs);
     std::copy(segs.begin(), segs.end(), args.segs);
     args.nouts = (int)outs.size();
+    if (tiles < SPLITK_TILES) {
+      int kmax = 0;
+      for (const auto& h : outs) {
+        int kt = 0;
+        for (int i = 0; i < h.nseg; ++i) kt += segs[h.seg0 + i].k;
+        kmax = std::max(kmax, kt);
+      }
+      parts = std::max(1, std::min((2 * SPLITK_TILES + tiles - 1) / tiles, kmax / SPLITK_MIN_K));
+    }
   } else {
     d_outs = (const GemmOut<T>*)cx.push(outs.data(), sizeof(GemmOut<T>) * outs.size());
     d_segs = (const GemmSeg<T>*)cx.push(segs.data(), sizeof(GemmSeg<T>) * segs.size());
   }
+  T* partial = parts > 1 ? (T*)cx.pool.alloc(sizeof(T) * (size_t)tiles * parts * GT * GT) : nullptr;
   // the profiled interval holds the kernel alone (the parameter copies come before it)
   const int h = prof_tag >= 0 ? cx.prof.begin(cx.st) : -1;
-  if (small) {
+  if (parts > 1) {
+    hipLaunchKernelGGL((gemm_splitk_kernel<T>), dim3(tiles * parts), dim3(256), 0, cx.st, args, parts, partial);
+    hipLaunchKernelGGL((gemm_splitk_reduce_kernel<T>), dim3(tiles), dim3(256), 0, cx.st, args, parts,
+                       (const T*)partial);
+  } else if (small) {
     hipLaunchKernelGGL((gemm_small_kernel<T, 0>), dim3(tiles), dim3(256), 0, cx.st, args);
   } else if (role == 1) {
     hipLaunchKernelGGL((gemm_grouped_kernel<T, 1>), dim3(tiles), dim3(256), 0, cx.st, d_outs, (int)outs.size(), d_segs);
@@ -202,6 +287,7 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
     for (const auto& o : ops) fl += 8.0 * o.m * (double)o.n * std::max(o.k, 0);
     cx.prof.end(cx.st, h, prof_tag, fl);
   }
+  if (partial) cx.pool.free(partial);  // stream ordered: later users of the block run after the reduce
   HIPCHK(hipGetLastError());
 }
 

@@ -48,6 +48,7 @@ RefineResult gmres_refine(SolverBase& op, SolverBase& pre, const cpx<double>* b_
   if (pre.size() != N) throw std::runtime_error("refine: operator and factorization differ in size");
   if (restart < 1) restart = 30;
   RefineResult res;
+  pre.prepare_repeated_solves();
   DMat<Z> V = dmat(cx, N, restart + 1), r = dmat(cx, N, 1), w = dmat(cx, N, 1), zt = dmat(cx, N, 1);
   DMat<Z> h = dmat(cx, restart + 1, 2);  // two CGS passes
   for (int c = 0; c < nrhs; ++c) {

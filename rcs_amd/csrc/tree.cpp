@@ -62,6 +62,8 @@ namespace {
 template <typename T>
 struct Box {
   bool ILActive = true;
+  // L2P has orthonormal columns / P2M orthonormal rows (set when they come from an RRQR's Q)
+  bool orthL2P = false, orthP2M = false;
   double maxPivot_L2P = 0.0, maxPivot_P2M = 0.0;
   double cx = 0, cy = 0, cz = 0;
   std::vector<int> IL, NB;
@@ -73,6 +75,7 @@ struct Box {
   DMat<T> luP2P, luL2M;
   int* ipP2P = nullptr;
   int* ipL2M = nullptr;
+  DMat<T> invP2P, invL2M;  // inverses of the factored blocks, formed for repeated solves
   // solve vectors (n x nrhs)
   DMat<T> particles, multipoles, locals, particle_rhs, multipole_rhs, local_rhs;
 };
@@ -209,6 +212,36 @@ class Solver : public SolverBase {
   int* upload_ints(const std::vector<int>& v) {
     if (v.empty()) return nullptr;
     return (int*)cx.push(v.data(), sizeof(int) * v.size());
+  }
+  // A^{-1} rhs as a new block: a product with the stored inverse when repeated solves were
+  // prepared, else triangular solves with the LU factors.
+  DMat<T> solved(DMat<T> lu, const int* ip, DMat<T> inv, DMat<T> rhs) {
+    if (inv.p) {
+      stats_.getrs_flops += 8.0 * inv.r * (double)inv.c * rhs.c;
+      return mul(inv, OP_N, rhs, OP_N);
+    }
+    DMat<T> s = clone(rhs);
+    solve_inplace(lu, ip, s);
+    return s;
+  }
+  DMat<T> inverse_of(DMat<T> lu, const int* ip) {
+    DMat<T> inv = newmat(lu.r, lu.r);
+    fill_identity(inv, T(1, 0), cx.st);
+    solve_inplace(lu, ip, inv);
+    return inv;
+  }
+  bool inverses_ready_ = false;
+  void prepare_repeated_solves() override {
+    if (!factorized_) throw std::runtime_error("prepare_repeated_solves before factorize");
+    if (inverses_ready_) return;
+    for (int j = startLevel; j <= nLevels; ++j)
+      for (int k = 0; k < nBoxes[j]; ++k) {
+        Box<T>& b = tree[j][k];
+        if (b.luP2P.p && b.luP2P.r > 0) put(b.invP2P, inverse_of(b.luP2P, b.ipP2P));
+        if (b.luL2M.p && b.luL2M.r > 0) put(b.invL2M, inverse_of(b.luL2M, b.ipL2M));
+      }
+    if (luBase.p && luBase.r > 0) put(invBase, inverse_of(luBase, ipBase));
+    inverses_ready_ = true;
   }
   void free_tmp_ints() {}
   // Blocks Z(rows_i, cols_i) for many pairs in one launch.
@@ -622,14 +655,19 @@ class Solver : public SolverBase {
 
   void assemble() override {
     double t0 = wall();
+    trace_t0 = t0;
     cx.pool.reset_peak();
     setup();
     trace("setup: N=%d nLevels=%d", N, nLevels);
     getNodes();
     trace("skeletons done, startLevel=%d", startLevel);
     assemble_M2L();
+    if (verbose) cx.sync();
+    trace("M2L blocks done");
     free_points();
     initialise_phase();
+    if (verbose) cx.sync();
+    trace("L2P/P2M done");
     initialise_P2P_Leaf_Level();
     cx.sync();
     trace("assembly done");
@@ -896,30 +934,70 @@ class Solver : public SolverBase {
     double maxPivot;
   };
   // Independent RRQRs (same level tolerances and fill-in cap) through shared launches.
+  //
+  // orth[i]: the first parts[i] columns of As[i] are known to be orthonormal (the Q of an
+  // earlier RRQR). The first partition's pivoted QR then only re-derives their span: every
+  // pivot is 1 to working precision, so it neither stops early nor truncates. For such a
+  // matrix the first partition is taken as it is, and the second partition's RRQR runs on
+  // the remaining columns projected twice onto the complement of that span. Its pivots, the
+  // stopping test on them and the fill-in cap are those of the second partition of the
+  // full RRQR; the norms it pivots on are computed from the projected columns instead of
+  // downdated (the same values up to rounding). Q = [leading columns | new columns].
   std::vector<QR> do_rrqr_batch(const std::vector<DMat<T>>& As, const std::vector<int>& parts,
-                                const std::vector<double>& pivots, int j) {
+                                const std::vector<double>& pivots, int j, const std::vector<char>& orth) {
     const size_t nb = As.size();
     std::vector<RRQRTask<T>> tk(nb);
-    for (size_t i = 0; i < nb; ++i) {
-      tk[i].A = As[i];
-      tk[i].partition = parts[i];
-      tk[i].inputPivot = pivots[i];
-      tk[i].fillInRank = rankVec[j];
-      tk[i].Q = newmat(As[i].r, std::max(1, std::min(As[i].r, As[i].c)));
-    }
+    std::vector<char> proj(nb, 0);
+    std::vector<DMat<T>> Pp(nb), C(nb);
     double t0 = wall();
+    for (size_t i = 0; i < nb; ++i) {
+      const int r = parts[i], m = As[i].r, n = As[i].c;
+      proj[i] = orth[i] && r > 0 && r < m && r < n && !full_rrqr_;
+    }
+    for (int pass = 0; pass < 2; ++pass) {  // P <- P - Q1 (Q1^H P), twice
+      Group g1, g2;
+      for (size_t i = 0; i < nb; ++i) {
+        if (!proj[i]) continue;
+        const int r = parts[i];
+        DMat<T> Q1 = As[i].cols(0, r);
+        if (pass == 0) Pp[i] = clone(As[i].cols(r, As[i].c - r));
+        if (pass == 0) C[i] = newmat(r, Pp[i].c);
+        add(g1, C[i], Q1, OP_C, Pp[i], OP_N, T(1, 0), T(0, 0));
+        add(g2, Pp[i], Q1, OP_N, C[i], OP_N, T(-1, 0), T(1, 0));
+      }
+      run(g1);
+      run(g2);
+    }
+    for (size_t i = 0; i < nb; ++i) {
+      drop(C[i]);
+      const int m = As[i].r;
+      if (proj[i]) {
+        tk[i].A = Pp[i];
+        tk[i].partition = 0;
+        tk[i].fillInRank = std::min(rankVec[j], m - parts[i]);
+      } else {
+        tk[i].A = As[i];
+        tk[i].partition = parts[i];
+        tk[i].fillInRank = rankVec[j];
+      }
+      tk[i].inputPivot = pivots[i];
+      tk[i].Q = newmat(m, std::max(1, std::min(m, tk[i].A.c)));
+    }
     int h = cx.prof.begin(cx.st);
     rrqr_batched(tk, std::pow(10.0, -TOL_POW), std::pow(10.0, -fill_in_TOL_POW), cx);
     double fl = 0;
     std::vector<QR> out(nb);
     for (size_t i = 0; i < nb; ++i) {
       const RRQRInfo& inf = tk[i].info;
-      double r = inf.rank, m = As[i].r, n = As[i].c;
+      const int lead = proj[i] ? parts[i] : 0;
+      double r = inf.rank, m = tk[i].A.r, n = tk[i].A.c;
       fl += 8.0 * (2.0 * m * n * r - r * r * (m + n) + 2.0 * r * r * r / 3.0) + 8.0 * (2.0 * m * r * r - 2.0 * r * r * r / 3.0);
-      out[i].Q = newmat(As[i].r, inf.rank);
-      if (inf.rank > 0) copy2d(out[i].Q, tk[i].Q.cols(0, inf.rank), cx.st);
+      out[i].Q = newmat(As[i].r, lead + inf.rank);
+      if (lead > 0) copy2d(out[i].Q.cols(0, lead), As[i].cols(0, lead), cx.st);
+      if (inf.rank > 0) copy2d(out[i].Q.cols(lead, inf.rank), tk[i].Q.cols(0, inf.rank), cx.st);
       out[i].maxPivot = inf.maxPivot;
       drop(tk[i].Q);
+      if (proj[i]) drop(Pp[i]);
     }
     cx.prof.end(cx.st, h, PROF_RRQR, fl);
     stats_.t_compress += wall() - t0;
@@ -1001,6 +1079,7 @@ class Solver : public SolverBase {
     }
     copy_batched(cops, cx);
     put(par.L2P, Pn);
+    par.orthL2P = false;
     drop(Rm);
     drop(Uk);
   }
@@ -1023,6 +1102,7 @@ class Solver : public SolverBase {
     }
     copy_batched(cops, cx);
     put(par.P2M, Pn);
+    par.orthP2M = false;
     drop(Rm);
     drop(Vb);
   }
@@ -1041,7 +1121,8 @@ class Solver : public SolverBase {
       part[d] = br.L2P.c;
       piv[d] = br.maxPivot_L2P;
     }
-    std::vector<QR> q1 = do_rrqr_batch(A, part, piv, j);
+    std::vector<char> orth1{tree[j][ra[0]].orthL2P, tree[j][ra[1]].orthL2P};
+    std::vector<QR> q1 = do_rrqr_batch(A, part, piv, j, orth1);
     DMat<T> Rk[2], Rkp[2];
     for (int d = 0; d < 2; ++d) {
       Box<T>& br = tree[j][ra[d]];
@@ -1066,7 +1147,8 @@ class Solver : public SolverBase {
       part[d] = bc.P2M.r;
       piv[d] = bc.maxPivot_P2M;
     }
-    std::vector<QR> q2 = do_rrqr_batch(A, part, piv, j);
+    std::vector<char> orth2{tree[j][ca[0]].orthP2M, tree[j][ca[1]].orthP2M};
+    std::vector<QR> q2 = do_rrqr_batch(A, part, piv, j, orth2);
     DMat<T> Rn[2], Rnp[2], t[2];
     for (int d = 0; d < 2; ++d) {
       Box<T>& bc = tree[j][ca[d]];
@@ -1123,10 +1205,12 @@ class Solver : public SolverBase {
       update_parent_P2M_old(j, b, P2Mtb_m);
     }
     put(tree[j][k].L2P, clone(L2Pk_m));
+    tree[j][k].orthL2P = true;
     {
       DMat<T> t = newmat(P2Mtb_m.c, P2Mtb_m.r);
       adjoint2d(t, P2Mtb_m, cx.st);
       put(tree[j][b].P2M, t);
+      tree[j][b].orthP2M = true;
     }
     tree[j][k].NumLocals = tree[j][k].L2P.c;
     tree[j][b].NumMultipoles = tree[j][b].P2M.r;
@@ -1138,10 +1222,12 @@ class Solver : public SolverBase {
       update_parent_P2M_old(j, k, P2Mtk_m);
     }
     put(tree[j][b].L2P, clone(L2Pb_m));
+    tree[j][b].orthL2P = true;
     {
       DMat<T> t = newmat(P2Mtk_m.c, P2Mtk_m.r);
       adjoint2d(t, P2Mtk_m, cx.st);
       put(tree[j][k].P2M, t);
+      tree[j][k].orthP2M = true;
     }
     tree[j][b].NumLocals = tree[j][b].L2P.c;
     tree[j][k].NumMultipoles = tree[j][k].P2M.r;
@@ -1155,7 +1241,8 @@ class Solver : public SolverBase {
     // the column and row RRQRs read disjoint inputs: one batched launch sequence
     DMat<T> Acol = hcat2(bk.L2P, OP_N, bk.M2P[b], OP_N);
     DMat<T> Arow = hcat2(bk.P2M, OP_C, bb.P2L[k], OP_C);
-    std::vector<QR> qq = do_rrqr_batch({Acol, Arow}, {bk.L2P.c, bk.P2M.r}, {bk.maxPivot_L2P, bk.maxPivot_P2M}, j);
+    std::vector<QR> qq = do_rrqr_batch({Acol, Arow}, {bk.L2P.c, bk.P2M.r}, {bk.maxPivot_L2P, bk.maxPivot_P2M}, j,
+                                       {(char)bk.orthL2P, (char)bk.orthP2M});
     drop(Acol);
     drop(Arow);
     const QR& q1 = qq[0];
@@ -1186,6 +1273,7 @@ class Solver : public SolverBase {
     update_other_M2Ls_at_k_old(j, k, b, L2Pm);
     if (j > startLevel) update_parent_L2P_old(j, k, L2Pm);
     put(bk.L2P, clone(L2Pm));
+    bk.orthL2P = true;
     bk.NumLocals = bk.L2P.c;
     put(bb.M2L[k], clone(M2Lbk.cols(0, P2Mtm.c)));
     update_other_M2Ls_at_b_old(j, b, k, P2Mtm);
@@ -1194,6 +1282,7 @@ class Solver : public SolverBase {
       DMat<T> t = newmat(P2Mtm.c, P2Mtm.r);
       adjoint2d(t, P2Mtm, cx.st);
       put(bk.P2M, t);
+      bk.orthP2M = true;
     }
     bk.NumMultipoles = bk.P2M.r;
     drop(nL2P); drop(Rk); drop(M2Lkb); drop(nP2Mt); drop(Rn); drop(Rnp); drop(M2Lbk);
@@ -1439,8 +1528,7 @@ class Solver : public SolverBase {
         Box<T>& bk = tree[j][k];
         bk.Eliminated = true;
         if (bk.P2P[k].size() != 0) {  // T:3613-3653
-          DMat<T> s = clone(bk.particle_rhs);
-          solve_inplace(bk.luP2P, bk.ipP2P, s);
+          DMat<T> s = solved(bk.luP2P, bk.ipP2P, bk.invP2P, bk.particle_rhs);
           Group g;
           put(bk.local_rhs, newmat(bk.P2M.r, nrhs));
           add(g, bk.local_rhs, bk.P2M, OP_N, s, OP_N, T(-1, 0), T(0, 0));
@@ -1456,8 +1544,7 @@ class Solver : public SolverBase {
           put(bk.local_rhs, zeros(bk.P2M.r, nrhs));
         }
         if (bk.L2M_f.size() > 0) {  // T:4115-4148
-          DMat<T> s = clone(bk.local_rhs);
-          solve_inplace(bk.luL2M, bk.ipL2M, s);
+          DMat<T> s = solved(bk.luL2M, bk.ipL2M, bk.invL2M, bk.local_rhs);
           Group g;
           for (int nn : bk.NB) {
             if (nn == k) continue;
@@ -1485,7 +1572,7 @@ class Solver : public SolverBase {
           ri += tree[j][r].NumLocals;
         }
         copy_batched(cops, cx);
-        solve_inplace(luBase, ipBase, b);
+        put(b, solved(luBase, ipBase, invBase, b));
         int idx = 0;
         for (int k = 0; k < nBoxes[j]; ++k) {
           int nm = tree[j][k].NumMultipoles;
@@ -1510,8 +1597,7 @@ class Solver : public SolverBase {
             else add(g, rhs1, bk.M2P[nn], OP_N, bn.multipoles, OP_N, T(-1, 0), T(1, 0), 1);
           }
           run(g);
-          DMat<T> t = clone(rhs1);
-          solve_inplace(bk.luP2P, bk.ipP2P, t);
+          DMat<T> t = solved(bk.luP2P, bk.ipP2P, bk.invP2P, rhs1);
           DMat<T> rhs2 = clone(bk.multipoles);
           {
             Group g2;
@@ -1519,8 +1605,8 @@ class Solver : public SolverBase {
             run(g2);
           }
           if (bk.L2M_f.size() != 0) {  // Atemp == L2M_f
-            solve_inplace(bk.luL2M, bk.ipL2M, rhs2);
-            put(bk.locals, rhs2);
+            put(bk.locals, solved(bk.luL2M, bk.ipL2M, bk.invL2M, rhs2));
+            drop(rhs2);
           } else {
             drop(rhs2);
             put(bk.locals, newmat(0, nrhs));
@@ -1530,8 +1616,8 @@ class Solver : public SolverBase {
             add(g3, rhs1, bk.L2P, OP_N, bk.locals, OP_N, T(-1, 0), T(1, 0));
             run(g3);
           }
-          solve_inplace(bk.luP2P, bk.ipP2P, rhs1);
-          put(bk.particles, rhs1);
+          put(bk.particles, solved(bk.luP2P, bk.ipP2P, bk.invP2P, rhs1));
+          drop(rhs1);
           drop(t);
         } else {
           put(bk.particles, newmat(0, nrhs));
@@ -1656,16 +1742,19 @@ class Solver : public SolverBase {
   std::vector<float> rank_log_;
   std::vector<std::vector<Box<T>>> tree;
   std::vector<std::pair<int, int>> P2Plist, P2L_M2P;
-  DMat<T> luBase;
+  DMat<T> luBase, invBase;
   int* ipBase = nullptr;
   int NMb = 0, NLb = 0;
   // RCS_DEBUG_STOP="level:box" stops the factorization after that box (inspection only)
   int dbg_level = -1, dbg_box = -1;
   // RCS_VERBOSE=1 prints one line per phase to stderr (flushed)
   bool verbose = std::getenv("RCS_VERBOSE") != nullptr;
+  // RCS_FULL_RRQR=1 runs the full partitioned RRQR even when the leading block is orthonormal
+  bool full_rrqr_ = std::getenv("RCS_FULL_RRQR") != nullptr;
+  double trace_t0 = wall();
   void trace(const char* fmt, int a = 0, int b = 0) {
     if (!verbose) return;
-    std::fprintf(stderr, "[rcs] ");
+    std::fprintf(stderr, "[rcs %8.3f s] ", wall() - trace_t0);
     std::fprintf(stderr, fmt, a, b);
     std::fprintf(stderr, " (pool %.2f GiB)\n", cx.pool.in_use() / 1073741824.0);
     std::fflush(stderr);

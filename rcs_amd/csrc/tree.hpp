@@ -68,6 +68,10 @@ class SolverBase {
   virtual void solve_device(const void* rhs, int nrhs, void* x, void* x_box) = 0;
   // y = Z x with the assembled compressed operator (before factorize), device arrays.
   virtual void apply_device(const void* x, int nrhs, void* y) = 0;
+  // Before many solves with few right-hand sides (iterative refinement): form the inverses
+  // of the factored diagonal blocks once, so that every later solve multiplies by them
+  // instead of running triangular solves.
+  virtual void prepare_repeated_solves() {}
   virtual int size() const = 0;
   virtual Ctx& context() = 0;
   // rows of RANK_LOG_COLS floats, one per ACA problem of the assembly
