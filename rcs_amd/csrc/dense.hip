@@ -1,5 +1,6 @@
 // Dense complex kernels for gfx950 (CDNA4, 64-wide wavefronts).
 #include <algorithm>
+#include <cstdlib>
 
 #include "dense.hpp"
 
@@ -205,6 +206,62 @@ __global__ void __launch_bounds__(256) gemm_splitk_reduce_kernel(GemmArgs<T> a, 
   gemm_store(out, m0, n0, acc);
 }
 
+// Narrow products (at most NARROW_N output columns, op(A) = A, op(B) = B), as in solves with
+// few right-hand sides: one output row per lane, the four wavefronts of a workgroup split
+// the inner index and add their parts in LDS in a fixed order. A is read once, coalesced.
+constexpr int NARROW_N = 8, NA_OUTS = 32, NA_SEGS = 32;
+template <typename T>
+struct NarrowArgs {
+  GemmOut<T> outs[NA_OUTS];  // tile0: first 64-row block of the output
+  GemmSeg<T> segs[NA_SEGS];
+  int nouts;
+};
+template <typename T>
+__global__ void __launch_bounds__(256) gemm_narrow_kernel(NarrowArgs<T> a) {
+  const int blk = blockIdx.x;
+  int o = 0;
+  for (int q = 1; q < a.nouts; ++q)
+    if (a.outs[q].tile0 <= blk) o = q;
+  const GemmOut<T>& out = a.outs[o];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = (blk - out.tile0) * 64 + lane, n = out.n;
+  T acc[NARROW_N];
+#pragma unroll
+  for (int c = 0; c < NARROW_N; ++c) acc[c] = T(0, 0);
+  for (int sgi = 0; sgi < out.nseg; ++sgi) {
+    const GemmSeg<T>& sg = a.segs[out.seg0 + sgi];
+    if (r < out.m) {
+      const T* arow = sg.A + r;
+#pragma unroll 4
+      for (int k = wave; k < sg.k; k += 4) {
+        const T av = arow[(size_t)k * sg.lda];
+#pragma unroll
+        for (int c = 0; c < NARROW_N; ++c)
+          if (c < n) cfma(acc[c], av, sg.B[k + (size_t)c * sg.ldb]);
+      }
+    }
+  }
+  __shared__ T part[3][NARROW_N][64];
+  if (wave > 0) {
+#pragma unroll
+    for (int c = 0; c < NARROW_N; ++c) part[wave - 1][c][lane] = acc[c];
+  }
+  __syncthreads();
+  if (wave == 0 && r < out.m) {
+    using R = typename RealOf<T>::type;
+    const bool zb = (out.beta.re == R(0) && out.beta.im == R(0));
+#pragma unroll
+    for (int c = 0; c < NARROW_N; ++c) {
+      if (c >= n) continue;
+      const T v0 = ((acc[c] + part[0][c][lane]) + part[1][c][lane]) + part[2][c][lane];
+      T* cp = out.C + (size_t)r + (size_t)c * out.ldc;
+      T v = out.alpha * v0;
+      if (!zb) v = v + out.beta * (*cp);
+      *cp = v;
+    }
+  }
+}
+
 constexpr int SPLITK_TILES = 96;    // groups with fewer output tiles than this split the inner index
 constexpr int SPLITK_MIN_K = 64;    // inner index per part, at least
 
@@ -245,6 +302,31 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
       throw std::runtime_error("gemm_grouped: two output blocks of one launch share C (chain them)");
   }
   if (segs.empty()) segs.push_back(GemmSeg<T>{});
+  static const bool no_narrow = std::getenv("RCS_NO_NARROW") != nullptr;  // debugging: tile kernel only
+  bool narrow = !no_narrow && role == 0 && outs.size() <= (size_t)NA_OUTS && segs.size() <= (size_t)NA_SEGS;
+  for (const auto& h : outs) narrow = narrow && h.n <= NARROW_N;
+  for (const auto& sg : segs) narrow = narrow && sg.opA == OP_N && sg.opB == OP_N;
+  if (narrow) {
+    static_assert(sizeof(NarrowArgs<T>) <= 4000, "narrow GEMM arguments exceed the kernel argument segment");
+    NarrowArgs<T> na;
+    int blocks = 0;
+    for (size_t i = 0; i < outs.size(); ++i) {
+      na.outs[i] = outs[i];
+      na.outs[i].tile0 = blocks;
+      blocks += (outs[i].m + 63) / 64;
+    }
+    std::copy(segs.begin(), segs.end(), na.segs);
+    na.nouts = (int)outs.size();
+    const int h = prof_tag >= 0 ? cx.prof.begin(cx.st) : -1;
+    hipLaunchKernelGGL((gemm_narrow_kernel<T>), dim3(blocks), dim3(256), 0, cx.st, na);
+    if (h >= 0) {
+      double fl = 0;
+      for (const auto& o : ops) fl += 8.0 * o.m * (double)o.n * std::max(o.k, 0);
+      cx.prof.end(cx.st, h, prof_tag, fl);
+    }
+    HIPCHK(hipGetLastError());
+    return;
+  }
   // (Schur updates keep one kernel name in traces: always the HBM parameter tables)
   const bool small = role == 0 && outs.size() <= (size_t)GA_OUTS && segs.size() <= (size_t)GA_SEGS;
   GemmArgs<T> args;
@@ -480,55 +562,78 @@ void getrf(DMat<T> A, int* ipiv, Ctx& cx) {
 // Few right-hand sides of a system of at most GETRS_LDS unknowns: one workgroup per
 // column, the column gathered through the permutation into LDS and solved there.
 constexpr int GETRS_LDS = 1024;
+// Solve with the LU factors for a few right-hand sides (one workgroup per column), in blocks
+// of 64 unknowns: the block's diagonal part of L (or U) is staged in LDS by all threads, one
+// wavefront solves it (lane i owns unknown i, the solved value is broadcast by a shuffle),
+// then all threads update the remaining unknowns with the block's columns (coalesced reads).
+// x_r - sum_{k < nb} LU(r, j0 + k) x_{j0 + k}, with sixteen loads of the row issued together.
+template <typename T>
+__device__ __forceinline__ T getrs_update(const T* LU, int ld, int r, int j0, int nb, const T* xl) {
+  T s = xl[r];
+  int k = 0;
+  for (; k + 16 <= nb; k += 16) {
+    T a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = LU[(size_t)(j0 + k + q) * ld + r];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s = s - a[q] * xl[j0 + k + q];
+  }
+  for (; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xl[j0 + k];
+  return s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) getrs_vec_kernel(const T* LU, int ld, int n, const int* perm, T* B, int ldb) {
   __shared__ T xl[GETRS_LDS];
+  __shared__ T dg[64][65];  // dg[k][i] = LU(j0 + i, j0 + k)
   T* b = B + (size_t)blockIdx.x * ldb;
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += blockDim.x) xl[i] = b[perm[i]];
-  __syncthreads();
-  // forward (unit lower), blocks of 64: one wavefront solves the diagonal block, then all
-  // threads update the rows below
+  // forward (unit lower)
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int nb = min(64, n - j0);
+#pragma unroll 16
+    for (int e = tid; e < nb * 64; e += blockDim.x) {
+      const int k = e >> 6, i = e & 63;
+      dg[k][i] = i < nb ? LU[(size_t)(j0 + k) * ld + j0 + i] : T(0, 0);
+    }
+    __syncthreads();
     if (tid < 64) {
       T x = tid < nb ? xl[j0 + tid] : T(0, 0);
       for (int k = 0; k < nb; ++k) {
         T xk;
         xk.re = __shfl(x.re, k);
         xk.im = __shfl(x.im, k);
-        if (tid > k && tid < nb) x = x - LU[(size_t)(j0 + k) * ld + j0 + tid] * xk;
+        if (tid > k && tid < nb) x = x - dg[k][tid] * xk;
       }
       if (tid < nb) xl[j0 + tid] = x;
     }
     __syncthreads();
-    for (int r = j0 + nb + tid; r < n; r += blockDim.x) {
-      T s = xl[r];
-      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xl[j0 + k];
-      xl[r] = s;
-    }
+    for (int r = j0 + nb + tid; r < n; r += blockDim.x) xl[r] = getrs_update(LU, ld, r, j0, nb, xl);
     __syncthreads();
   }
   // backward (upper)
   for (int j1 = n; j1 > 0; j1 -= 64) {
     const int j0 = max(0, j1 - 64), nb = j1 - j0;
+#pragma unroll 16
+    for (int e = tid; e < nb * 64; e += blockDim.x) {
+      const int k = e >> 6, i = e & 63;
+      dg[k][i] = i < nb ? LU[(size_t)(j0 + k) * ld + j0 + i] : T(0, 0);
+    }
+    __syncthreads();
     if (tid < 64) {
       T x = tid < nb ? xl[j0 + tid] : T(0, 0);
       for (int k = nb - 1; k >= 0; --k) {
-        if (tid == k) x = cdiv(x, LU[(size_t)(j0 + k) * ld + j0 + k]);
+        if (tid == k) x = cdiv(x, dg[k][k]);
         T xk;
         xk.re = __shfl(x.re, k);
         xk.im = __shfl(x.im, k);
-        if (tid < k) x = x - LU[(size_t)(j0 + k) * ld + j0 + tid] * xk;
+        if (tid < k) x = x - dg[k][tid] * xk;
       }
       if (tid < nb) xl[j0 + tid] = x;
     }
     __syncthreads();
-    for (int r = tid; r < j0; r += blockDim.x) {
-      T s = xl[r];
-      for (int k = 0; k < nb; ++k) s = s - LU[(size_t)(j0 + k) * ld + r] * xl[j0 + k];
-      xl[r] = s;
-    }
+    for (int r = tid; r < j0; r += blockDim.x) xl[r] = getrs_update(LU, ld, r, j0, nb, xl);
     __syncthreads();
   }
   for (int i = tid; i < n; i += blockDim.x) b[i] = xl[i];

@@ -1,9 +1,12 @@
 // FP64 iterative refinement of a (possibly complex64) AIFMM factorization: restarted
-// GMRES on Z x = b, right-preconditioned by the factorization's solve, with the residual
-// formed by a separately assembled FP64 compressed operator. Classical Gram-Schmidt run
-// twice (CGS2) orthogonalises each new Krylov vector with two products against the basis.
-// All vectors live in HBM; the host only keeps the small Hessenberg system and its Givens
-// rotations (one synchronisation per iteration).
+// flexible GMRES on Z x = b, right-preconditioned by the factorization's solve, with the
+// residual formed by a separately assembled FP64 compressed operator. The preconditioned
+// vectors z_k = M^{-1} v_k are kept and the update is x += [z_0 .. z_k] y: a complex64
+// solve is linear only up to its rounding, which plain right-preconditioned GMRES (update
+// M^{-1} (V y)) assumes exact. Classical Gram-Schmidt run twice (CGS2) orthogonalises each
+// new Krylov vector with two products against the basis. All vectors live in HBM; the host
+// only keeps the small Hessenberg system and its Givens rotations (one synchronisation per
+// iteration).
 //
 // No counterpart in the reference (its driver checks a dense-LU residual, test/sphere.cpp:
 // 331-338); this is the refinement step of the mixed-precision path.
@@ -49,7 +52,7 @@ RefineResult gmres_refine(SolverBase& op, SolverBase& pre, const cpx<double>* b_
   if (restart < 1) restart = 30;
   RefineResult res;
   pre.prepare_repeated_solves();
-  DMat<Z> V = dmat(cx, N, restart + 1), r = dmat(cx, N, 1), w = dmat(cx, N, 1), zt = dmat(cx, N, 1);
+  DMat<Z> V = dmat(cx, N, restart + 1), Zk = dmat(cx, N, restart), r = dmat(cx, N, 1), w = dmat(cx, N, 1);
   DMat<Z> h = dmat(cx, restart + 1, 2);  // two CGS passes
   for (int c = 0; c < nrhs; ++c) {
     DMat<Z> b{const_cast<Z*>(b_dev) + (size_t)c * N, N, 1, N};
@@ -74,9 +77,9 @@ RefineResult gmres_refine(SolverBase& op, SolverBase& pre, const cpx<double>* b_
       g[0] = beta;
       int k = 0;
       for (; k < restart && it < maxit; ++k, ++it) {
-        // w = Z M^{-1} v_k
-        pre.solve_device(V.cols(k, 1).p, 1, zt.p, nullptr);
-        op.apply_device(zt.p, 1, w.p);
+        // z_k = M^{-1} v_k, w = Z z_k
+        pre.solve_device(V.cols(k, 1).p, 1, Zk.cols(k, 1).p, nullptr);
+        op.apply_device(Zk.cols(k, 1).p, 1, w.p);
         // CGS2 against v_0..v_k
         DMat<Z> Vk = V.cols(0, k + 1);
         DMat<Z> h1 = h.cols(0, 1).rows(0, k + 1), h2 = h.cols(1, 1).rows(0, k + 1);
@@ -113,7 +116,7 @@ RefineResult gmres_refine(SolverBase& op, SolverBase& pre, const cpx<double>* b_
         g[k] = std::conj(cs[k]) * g[k];
         if (std::abs(g[k + 1]) <= tol * bn || wn == 0) { ++k; ++it; break; }
       }
-      // y = H^{-1} g (upper triangular k x k), x += M^{-1} V y
+      // y = H^{-1} g (upper triangular k x k), x += [z_0 .. z_{k-1}] y
       std::vector<cd> y(k);
       for (int i = k - 1; i >= 0; --i) {
         cd s = g[i];
@@ -124,16 +127,14 @@ RefineResult gmres_refine(SolverBase& op, SolverBase& pre, const cpx<double>* b_
       for (int i = 0; i < k; ++i) yz[i] = Z(y[i].real(), y[i].imag());
       DMat<Z> yd = h.cols(0, 1).rows(0, k);
       cx.upload(yd.p, yz.data(), sizeof(Z) * k);
-      gemm1(cx, r, V.cols(0, k), OP_N, yd, Z(1, 0), Z(0, 0));
-      pre.solve_device(r.p, 1, zt.p, nullptr);
-      axpy2d(x, zt, Z(1, 0), cx.st);
+      gemm1(cx, x, Zk.cols(0, k), OP_N, yd, Z(1, 0), Z(1, 0));
       ++res.restarts;
     }
     res.iters = std::max(res.iters, it);
     res.relres = std::max(res.relres, rel);
   }
   cx.sync();
-  for (DMat<Z>* m : {&V, &r, &w, &zt, &h}) cx.pool.free(m->p);
+  for (DMat<Z>* m : {&V, &Zk, &r, &w, &h}) cx.pool.free(m->p);
   return res;
 }
 

@@ -18,8 +18,11 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("opA,opB", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200),
+                                   (300, 1, 700), (65, 8, 129), (100, 100, 900)])
 def test_gemm(rng, opA, opB, m, n, k):
+    """Covers the tile kernel, the split inner index (few tiles, long k) and the narrow
+    kernel (n <= 8, no transposes)."""
     A = rng.normal(size=(m, k) if opA == 0 else (k, m)) + 1j * rng.normal(size=(m, k) if opA == 0 else (k, m))
     B = rng.normal(size=(k, n) if opB == 0 else (n, k)) + 1j * rng.normal(size=(k, n) if opB == 0 else (n, k))
     ref = (A if opA == 0 else A.conj().T) @ (B if opB == 0 else B.conj().T)
@@ -208,6 +211,25 @@ def test_aifmm_compressed_matches_oracle(kind, freq, tp, slack):
         eg, eo = _rel(xg, xd), _rel(xo, xd)
         print(f"  vs dense LU: gpu {eg:.3e} oracle {eo:.3e}")
         assert eg < 2 * eo + 1e-12
+
+
+def test_rrqr_orthonormal_lead_matches_full_rrqr(monkeypatch):
+    """Fill-in recompression whose leading block is the Q of an earlier RRQR runs only the
+    second partition on the projected columns; the full partitioned RRQR (RCS_FULL_RRQR=1)
+    must give the same ranks and the same solution to rounding."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    out = {}
+    for full in (0, 1):
+        if full:
+            monkeypatch.setenv("RCS_FULL_RRQR", "1")
+        g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+        s = R.AIFMM(g, M.bounding_box(g.locations()), 64, 5, 5, 0, R.C128)
+        s.factorize()
+        st = s.stats()
+        out[full] = (s.solve(g.rhs()), s.ranks(st["nLevels"]), st["n_compress_p2p"] + st["n_compress_p2l"])
+    assert out[0][2] == out[1][2] > 0
+    assert np.array_equal(out[0][1], out[1][1])
+    assert _rel(out[0][0], out[1][0]) < 1e-8
 
 
 def test_aifmm_multi_rhs_and_c64(rng):
