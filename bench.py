@@ -31,7 +31,7 @@ CONFIGS = {
     # name: (subdivision level, radius, freq, tol_pow, precision)
     "efie-sphere-30720": (5, 0.15, 1.0e9, 6, 0),
     "efie-sphere-7680": (4, 0.15, 1.0e9, 6, 0),
-    "efie-sphere-122880-c64": (6, 0.15, 3.82e9, 6, 1),
+    "efie-sphere-122880": (6, 0.15, 3.82e9, 6, 0),
 }
 
 
@@ -110,8 +110,10 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=3)
     ap.add_argument("--no-mixed", action="store_true", help="skip the accuracy / mixed-precision comparison")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank training and ablation")
-    ap.add_argument("--mixed-tol", type=int, default=4, help="TOL_POW of the complex64 factorization")
+    ap.add_argument("--mixed-tol", type=int, default=4, help="TOL_POW of the loose factorizations refined by GMRES")
     ap.add_argument("--plain-tol-accurate", type=int, default=9, help="TOL_POW of the accurate plain FP64 run")
+    ap.add_argument("--headline", action="store_true",
+                    help="only the refined-forms comparison on the 122,880-unknown sphere at 3.82 GHz")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -122,6 +124,9 @@ def main():
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.headline:
+        headline(args)
+        return
 
     import rcs_amd as R
     from rcs_amd import mesh as M
@@ -210,37 +215,44 @@ def main():
     }
     own_log = s.rank_log()
     del s
-    if not args.no_mixed:
-        out.update(accuracy_and_mixed(R, M, g, bb, b, x, freq, args, log))
+    net = None
     if not args.no_learned:
-        out["learned_rank"] = learned_rank(R, M, own_log, log)
+        net, out["learned_rank"] = learned_rank(R, M, own_log, log)
+    if not args.no_mixed:
+        out.update(accuracy_and_refined(R, g, bb, b, x, args, log, net))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
     print(json.dumps(out))
 
 
-def learned_rank(R, M, own_log, log):
-    """Train the rank predictor on ACA ranks logged from the Laplace sphere, the EFIE sphere
-    (including this benchmark's own assembly) and seeded ellipsoids, then assemble the
-    perturbed-ellipsoid workload (122,880 unknowns, 2 % radial noise, ka = 12) without and
-    with it: ACA host checks and time, assembly time, miss rate (boxes whose rank exceeded
-    the prediction), median relative rank error, and whether the skeletons are identical."""
+def train_rank_model(R, own_log, log):
+    """The rank predictor trained on ACA ranks logged from the Laplace sphere, the EFIE sphere
+    at three tolerances, seeded ellipsoids and (when given) this benchmark's own assembly."""
     from rcs_amd import rank_model as RM
     log("learned rank: collecting and training")
     t0 = time.perf_counter()
     X, t = RM.collect(RM.training_configs())
-    X = np.vstack([X, own_log[:, :8]])
-    t = np.concatenate([t, np.log1p(own_log[:, 8])]).astype(np.float32)
+    if own_log is not None:
+        X = np.vstack([X, own_log[:, :8]])
+        t = np.concatenate([t, np.log1p(own_log[:, 8])]).astype(np.float32)
     t1 = time.perf_counter()
     net, loss0, loss = RM.train(X, t, seed=1, iters=2000)
     t2 = time.perf_counter()
+    return net, {"training_samples": int(len(X)), "collect_s": t1 - t0, "train_s": t2 - t1, "train_iters": 2000,
+                 "loss_initial": loss0, "loss_final": loss}
+
+
+def learned_rank(R, M, own_log, log):
+    """Train the rank predictor, then assemble the perturbed-ellipsoid workload (122,880
+    unknowns, 2 % radial noise, ka = 12) without and with it: ACA host checks and time,
+    assembly time, miss rate (boxes whose rank exceeded the prediction), median relative
+    rank error, and whether the skeletons are identical."""
+    net, res = train_rank_model(R, own_log, log)
     freq = 3.82e9
     nodes, tris = M.perturbed_ellipsoid(6, (0.15, 0.12, 0.10), 0.02, seed=20261203)
     g = R.Mesh(nodes, tris, R.EFIE, freq)
     bb = M.bounding_box(g.locations())
-    res = {"training_samples": int(len(X)), "collect_s": t1 - t0, "train_s": t2 - t1, "train_iters": 2000,
-           "loss_initial": loss0, "loss_final": loss,
-           "workload": f"perturbed ellipsoid (0.15, 0.12, 0.10) m, 2% radial noise, N={g.N}, f={freq:g} Hz, TOL 6"}
+    res["workload"] = f"perturbed ellipsoid (0.15, 0.12, 0.10) m, 2% radial noise, N={g.N}, f={freq:g} Hz, TOL 6"
     runs = {}
     for name, model in (("adaptive", None), ("learned", net)):
         log(f"learned rank: assembling the perturbed ellipsoid ({name})")
@@ -260,17 +272,55 @@ def learned_rank(R, M, own_log, log):
                 "median_rel_rank_error": float(np.median(rel)),
                 "skeleton_ranks_identical": bool(np.array_equal(l0[:, 8], l1[:, 8])),
                 "aca_speedup": a0["seconds"] / max(a1["seconds"], 1e-12)})
-    return res
+    return net, res
 
 
-def accuracy_and_mixed(R, M, g, bb, b, x_plain, freq, args, log):
-    """Accuracy of the plain path and the mixed-precision form on the same workload.
+def refined_forms(R, g, bb, b, forms, op_ref, op_check, log, net, sols=None):
+    """Each form: factorization (precision, TOL_POW, with or without the rank predictor)
+    refined by flexible GMRES against the FP64 operator op_ref to a relative residual of
+    1e-6; its total time covers assembly, factorization and refinement (op_ref's own
+    assembly is shared and reported apart). The solution's residual is checked against
+    op_check (a tighter FP64 operator) when given; the solutions go to sols when given."""
+    rel = lambda r: float(np.linalg.norm(r) / np.linalg.norm(b))
+    out = {}
+    for name, prec, tp, learned in forms:
+        log(f"refined form {name}")
+        R.set_rank_model(net if learned else None)
+        try:
+            t0 = time.perf_counter()
+            s = R.AIFMM(g, bb, 64, tp, tp, 0, prec)
+            t1 = time.perf_counter()
+        finally:
+            R.set_rank_model(None)
+        s.factorize()
+        t2 = time.perf_counter()
+        x, it, rr = s.refine(op_ref, b, tol=1e-6, maxit=60, restart=30)
+        t3 = time.perf_counter()
+        out[name] = {"precision": "complex64" if prec == R.C64 else "complex128", "tol_pow": tp,
+                     "rank_predictor": bool(learned), "assembly_s": t1 - t0, "factor_s": t2 - t1,
+                     "refine_s": t3 - t2, "gmres_iterations": it, "relres_gmres": rr, "total_s": t3 - t0}
+        if op_check is not None:
+            out[name]["relres_check"] = rel(b - op_check.apply(x))
+        if sols is not None:
+            sols[name] = x
+        del s
+    base = out.get("plain_fp64_refined")
+    if base is not None:
+        for name, v in out.items():
+            ok = v["relres_gmres"] <= 1e-6 and base["relres_gmres"] <= 1e-6
+            v["speedup_vs_plain_fp64_refined"] = base["total_s"] / v["total_s"] if ok else None
+    return out
+
+
+def accuracy_and_refined(R, g, bb, b, x_plain, args, log, net):
+    """Accuracy of the plain path and the refined forms on the benchmark workload.
 
     The yardstick is the relative residual ||b - Z x|| / ||b|| with an FP64 compressed
-    operator at TOL_POW 10 (assembled once, timed separately). Mixed form: complex64
-    factorization at TOL_POW args.mixed_tol preconditioning FP64 GMRES to 1e-6; its time
-    includes its own operator assembly. The plain FP64 path is also run at TOL_POW
-    args.plain_tol_accurate to give a plain solution at comparable accuracy."""
+    operator at TOL_POW 10 (assembled once, timed separately). The plain FP64 direct solve
+    stops at the compression error (also at TOL_POW args.plain_tol_accurate), so every form
+    that reaches 1e-6 is refined by flexible GMRES against an FP64 TOL_POW 8 operator:
+    plain FP64 at TOL_POW 6 (the comparison base), FP64 at args.mixed_tol, and complex64 at
+    args.mixed_tol with the rank predictor (the mixed-precision form)."""
     rel = lambda r: float(np.linalg.norm(r) / np.linalg.norm(b))
     log("FP64 residual operator (TOL 10)")
     t0 = time.perf_counter()
@@ -278,34 +328,63 @@ def accuracy_and_mixed(R, M, g, bb, b, x_plain, freq, args, log):
     t_op = time.perf_counter() - t0
     res = {"residual_operator": {"tol_pow": 10, "assembly_s": t_op},
            "plain_relres": rel(b - op.apply(x_plain))}
-    # mixed: complex64 factorization + FP64 GMRES with its own (TOL 8) operator
-    log(f"mixed: complex64 factorization at TOL {args.mixed_tol} + GMRES")
-    t0 = time.perf_counter()
-    s32 = R.AIFMM(g, bb, 64, args.mixed_tol, args.mixed_tol, 0, R.C64)
-    t1 = time.perf_counter()
-    s32.factorize()
-    t2 = time.perf_counter()
-    opm = R.Operator(g, bb, 64, 8)
-    t3 = time.perf_counter()
-    xm, it, rr = s32.refine(opm, b, tol=1e-6, maxit=60, restart=30)
-    t4 = time.perf_counter()
-    res["mixed"] = {"precision": "complex64 factor + FP64 GMRES (CGS2)", "tol_pow_factor": args.mixed_tol,
-                    "tol_pow_operator": 8, "assembly_s": t1 - t0, "factor_s": t2 - t1, "operator_s": t3 - t2,
-                    "refine_s": t4 - t3, "gmres_iterations": it, "relres_gmres": rr,
-                    "relres_tol10": rel(b - op.apply(xm)), "total_s": t4 - t0}
-    del s32, opm
-    # plain FP64 at a tolerance that reaches the same accuracy class
     tp = args.plain_tol_accurate
     log(f"plain FP64 at TOL {tp}")
     t0 = time.perf_counter()
     sa = R.AIFMM(g, bb, 64, tp, tp, 0, R.C128)
     sa.factorize()
     xa = sa.solve(b)
-    t5 = time.perf_counter() - t0
-    res["plain_accurate"] = {"tol_pow": tp, "total_s": t5, "relres_tol10": rel(b - op.apply(xa))}
-    if res["mixed"]["relres_tol10"] <= 1e-6 and res["plain_accurate"]["relres_tol10"] <= 1e-6:
-        res["speedup_mixed_vs_plain_at_1e-6"] = t5 / res["mixed"]["total_s"]
+    res["plain_accurate"] = {"tol_pow": tp, "total_s": time.perf_counter() - t0, "relres_tol10": rel(b - op.apply(xa))}
+    del sa
+    log("FP64 refinement operator (TOL 8)")
+    t0 = time.perf_counter()
+    opm = R.Operator(g, bb, 64, 8)
+    res["refinement_operator"] = {"tol_pow": 8, "assembly_s": time.perf_counter() - t0}
+    forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
+             (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, net is not None)]
+    res["refined"] = refined_forms(R, g, bb, b, forms, opm, op, log, net)
+    del opm, op
     return res
+
+
+def headline(args):
+    """The refined forms on the 122,880-unknown sphere (icosphere level 6, r = 0.15 m,
+    f = 3.82 GHz, ka = 12): plain FP64 at TOL_POW 6, FP64 at args.mixed_tol and complex64 at
+    args.mixed_tol with the rank predictor, each refined by flexible GMRES to 1e-6 against
+    one FP64 TOL_POW 8 operator; residuals checked with an FP64 TOL_POW 10 operator."""
+    import rcs_amd as R
+    from rcs_amd import mesh as M
+
+    def log(msg):
+        print(f"[bench headline] {msg}", file=sys.stderr, flush=True)
+
+    nodes, tris = M.icosphere(6, 0.15)
+    freq = 3.82e9
+    g = R.Mesh(nodes, tris, R.EFIE, freq)
+    bb = M.bounding_box(g.locations())
+    b = g.rhs()
+    net, train = train_rank_model(R, None, log)
+    log("FP64 refinement operator (TOL 8)")
+    t0 = time.perf_counter()
+    opm = R.Operator(g, bb, 64, 8)
+    t_opm = time.perf_counter() - t0
+    forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
+             (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, True)]
+    sols = {}
+    ref = refined_forms(R, g, bb, b, forms, opm, None, log, net, sols)
+    del opm
+    log("FP64 check operator (TOL 10)")  # assembled after the forms: both operators and a
+    t0 = time.perf_counter()            # factorization together would not fit in HBM
+    op = R.Operator(g, bb, 64, 10)
+    t_op = time.perf_counter() - t0
+    for name, x in sols.items():
+        ref[name]["relres_check"] = float(np.linalg.norm(b - op.apply(x)) / np.linalg.norm(b))
+    del op
+    print(json.dumps({"workload": f"EFIE sphere r=0.15 m, icosphere level 6, N={g.N}, f={freq:g} Hz, leafSize=64, "
+                                  "1 plane-wave RHS, solution relative residual <= 1e-6",
+                      "refinement_operator": {"tol_pow": 8, "assembly_s": t_opm},
+                      "check_operator": {"tol_pow": 10, "assembly_s": t_op},
+                      "rank_predictor_training": train, "refined": ref}))
 
 
 if __name__ == "__main__":

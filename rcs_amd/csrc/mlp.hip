@@ -309,9 +309,9 @@ double RankMLP::train_host(const float* X, const float* t, int n, int iters, flo
 void RankMLP::predict_ranks(const float* X, int n, int* ranks) {
   std::vector<float> y(n);
   forward_host(X, n, y.data());
-  for (int i = 0; i < n; ++i) {
-    const double r = std::expm1((double)y[i]);
-    ranks[i] = r > 0 ? (int)std::ceil(r) : 1;
+  for (int i = 0; i < n; ++i) {  // the output is log(1 + rank); far outside the training data it can be any value
+    const double r = std::expm1(std::min(std::max((double)y[i], 0.0), 20.0));
+    ranks[i] = r > 0 ? (int)std::min(std::ceil(r), 1e8) : 1;
   }
 }
 

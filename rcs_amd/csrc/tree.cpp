@@ -428,7 +428,11 @@ class Solver : public SolverBase {
     std::vector<int> pred(which.size(), 0);
     if (P.rank_model && !which.empty()) {
       P.rank_model->predict_ranks(feat.data(), (int)which.size(), pred.data());
-      for (size_t i = 0; i < which.size(); ++i) probs[which[i]].pred = pred[i];
+      for (size_t i = 0; i < which.size(); ++i) {  // a rank never exceeds the short side
+        const AcaProblem& p = probs[which[i]];
+        pred[i] = std::max(1, std::min(pred[i], (int)std::min(p.rows.size(), p.cols.size())));
+        probs[which[i]].pred = pred[i];
+      }
     }
     AcaStats ast;
     ast.burst = P.aca_burst;

@@ -27,10 +27,14 @@ def assemble_and_log(nodes, tris, kind, freq, tol_pow, leaf_size=64):
 
 def training_configs(seed=2026, n_ellipsoids=6):
     """Small instances of the workload families the predictor is trained on: the Laplace
-    sphere (5,120 unknowns, TOL 8), the EFIE sphere at 1 GHz (7,680 unknowns, TOL 6), and
-    seeded ellipsoids with semi-axes in [0.05, 0.3] m at 1 GHz (subdivision 3, TOL 6)."""
-    cfg = [("sphere", 4, (0.15, 0.15, 0.15), LAPLACE, 1e9, 8),
-           ("sphere", 4, (0.15, 0.15, 0.15), EFIE, 1e9, 6)]
+    sphere (5,120 unknowns, TOL 8), the EFIE sphere at 1 GHz (7,680 unknowns at TOL 4, 6
+    and 8; 30,720 unknowns at TOL 4) and at 3.82 GHz (7,680 unknowns at TOL 4 and 6, the
+    electrical box sizes of the 122,880-unknown sphere's coarse levels), and seeded
+    ellipsoids with semi-axes in [0.05, 0.3] m at 1 GHz (subdivision 3, TOL 6)."""
+    cfg = [("sphere", 4, (0.15, 0.15, 0.15), LAPLACE, 1e9, 8)]
+    cfg += [("sphere", 4, (0.15, 0.15, 0.15), EFIE, 1e9, tp) for tp in (4, 6, 8)]
+    cfg += [("sphere", 5, (0.15, 0.15, 0.15), EFIE, 1e9, 4)]
+    cfg += [("sphere", 4, (0.15, 0.15, 0.15), EFIE, 3.82e9, tp) for tp in (4, 6)]
     rng = np.random.default_rng(seed)
     for _ in range(n_ellipsoids):
         axes = tuple(float(a) for a in rng.uniform(0.05, 0.3, size=3))
