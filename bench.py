@@ -267,6 +267,10 @@ def learned_rank(R, M, own_log, log):
         del s
     (a0, ta0, l0), (a1, ta1, l1) = runs["adaptive"], runs["learned"]
     rel = np.abs(l1[:, 9] - l1[:, 8]) / np.maximum(l1[:, 8], 1)
+    # bytes of V the ACA's long kernel streams over the whole assembly: at step k it reads the
+    # k earlier columns of the long side once (residual and Frobenius dots share the pass)
+    r, nl = l0[:, 8].astype(np.float64), l0[:, 13].astype(np.float64)
+    res["aca_long_v_bytes"] = float(np.sum(nl * r * (r + 1) / 2) * 16)
     res.update({"aca_host_checks": [a0["syncs"], a1["syncs"]], "aca_s": [a0["seconds"], a1["seconds"]],
                 "assembly_s": [ta0, ta1], "miss_rate": float(np.mean(l1[:, 8] > l1[:, 9])),
                 "median_rel_rank_error": float(np.median(rel)),
