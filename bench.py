@@ -373,7 +373,8 @@ def headline(args):
     opm = R.Operator(g, bb, 64, 8)
     t_opm = time.perf_counter() - t0
     forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
-             (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, True)]
+             (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, True),
+             (f"c64_tol{args.mixed_tol}_refined", R.C64, args.mixed_tol, False)]
     sols = {}
     ref = refined_forms(R, g, bb, b, forms, opm, None, log, net, sols)
     del opm

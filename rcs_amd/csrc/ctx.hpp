@@ -34,7 +34,7 @@ class DevPool {
       HIPCHK(hipMemGetInfo(&fr, &tot));
       limit_ = tot / 100 * 85;
     }
-    if (total_ + bytes > limit_) trim();
+    if (total_ + bytes > limit_) trim_to(limit_ > bytes ? limit_ - bytes : 0);
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -59,6 +59,15 @@ class DevPool {
   void trim() {
     for (auto& kv : free_) { (void)hipFree(kv.second); total_ -= kv.first; }
     free_.clear();
+  }
+  // Return cached blocks, largest first, until at most `target` bytes are held.
+  void trim_to(size_t target) {
+    while (total_ > target && !free_.empty()) {
+      auto it = std::prev(free_.end());
+      (void)hipFree(it->second);
+      total_ -= it->first;
+      free_.erase(it);
+    }
   }
   void release_all() {
     trim();

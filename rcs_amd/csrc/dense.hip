@@ -432,6 +432,97 @@ __global__ void __launch_bounds__(1024) lu_panel_kernel(T* A, int lda, int n, in
   }
 }
 
+// The same panel factorization with the panel in registers: thread t owns row j + t (all nb
+// <= LU_NB panel entries of it), so the pivot steps touch no global memory. The pivot row
+// and row c are exchanged through LDS, which also broadcasts the new U row. Needs
+// n - j <= NT; complex128 rows take 128 VGPRs (NT = 512), complex64 rows 64 (NT = 1024).
+// The steps are instantiated one by one (JJ a template argument) so that every index into
+// the row array is a constant and the array stays in registers.
+template <typename T, int NT>
+struct LuRegPanel {
+  using R = typename RealOf<T>::type;
+  R* sval;
+  int* sidx;
+  T* urow;
+  T* crow;
+  int* ipiv;
+  int n, j, nb, r, tid;
+  bool own;
+  template <int JJ>
+  __device__ __forceinline__ void step(T (&a)[LU_NB]) {
+    if (JJ < nb) {
+      const int lane = tid & 63, wave = tid >> 6, c = j + JJ;
+      R best = (own && r >= c) ? dabs(a[JJ]) : R(-1);
+      int bi = (own && r >= c) ? r : n;
+      for (int off = 32; off > 0; off >>= 1) {
+        R ov = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (lane == 0) { sval[wave] = best; sidx[wave] = bi; }
+      __syncthreads();
+      best = sval[0];
+      bi = sidx[0];
+#pragma unroll
+      for (int w = 1; w < NT / 64; ++w)
+        if (sval[w] > best || (sval[w] == best && sidx[w] < bi)) { best = sval[w]; bi = sidx[w]; }
+      const int p = (bi >= n) ? c : bi;
+      if (tid == 0) ipiv[c] = p;
+      if (r == p) {
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) urow[q] = a[q];
+      }
+      if (r == c) {
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) crow[q] = a[q];
+      }
+      __syncthreads();
+      if (p != c) {
+        if (r == c) {
+#pragma unroll
+          for (int q = 0; q < LU_NB; ++q) a[q] = urow[q];
+        } else if (r == p) {
+#pragma unroll
+          for (int q = 0; q < LU_NB; ++q) a[q] = crow[q];
+        }
+      }
+      const T piv = urow[JJ];
+      const bool nz = !(piv.re == R(0) && piv.im == R(0));
+      if (own && r > c) {
+        T l = a[JJ];
+        if (nz) l = cdiv(l, piv);
+        a[JJ] = l;
+#pragma unroll
+        for (int q = JJ + 1; q < LU_NB; ++q) a[q] = a[q] - l * urow[q];
+      }
+      __syncthreads();
+    }
+    if constexpr (JJ + 1 < LU_NB) step<JJ + 1>(a);
+  }
+};
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) lu_panel_reg_kernel(T* A, int lda, int n, int j, int nb, int* ipiv) {
+  using R = typename RealOf<T>::type;
+  __shared__ R sval[NT / 64];
+  __shared__ int sidx[NT / 64];
+  __shared__ T urow[LU_NB];  // the pivot row: row c after the exchange (its U part)
+  __shared__ T crow[LU_NB];  // row c before the exchange
+  LuRegPanel<T, NT> P{sval, sidx, urow, crow, ipiv, n, j, nb, j + (int)threadIdx.x, (int)threadIdx.x,
+                      j + (int)threadIdx.x < n};
+  T a[LU_NB];
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q) a[q] = (P.own && q < nb) ? A[(size_t)(j + q) * lda + P.r] : T(0, 0);
+  P.template step<0>(a);
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q)
+    if (P.own && q < nb) A[(size_t)(j + q) * lda + P.r] = a[q];
+}
+// register-panel thread count per element type
+template <typename T> struct LuRegNT;
+template <> struct LuRegNT<cpx<double>> { static constexpr int v = 512; };
+template <> struct LuRegNT<cpx<float>> { static constexpr int v = 1024; };
+
 // Apply the row interchanges of rows [j, j+nb) to the columns outside the panel.
 template <typename T>
 __global__ void laswp_kernel(T* A, int lda, int ncols, int j, int nb, const int* ipiv, int skip0, int skip1) {
@@ -536,10 +627,15 @@ void getrf(DMat<T> A, int* ipiv, Ctx& cx) {
   if (A.c != n) throw std::runtime_error("getrf: matrix is not square");
   for (int j = 0; j < n; j += LU_NB) {
     const int nb = std::min(LU_NB, n - j);
-    int threads = 1024;
-    if (n - j <= 256) threads = 256;
-    else if (n - j <= 512) threads = 512;
-    hipLaunchKernelGGL(lu_panel_kernel<T>, dim3(1), dim3(threads), 0, st, A.p, A.ld, n, j, nb, ipiv);
+    constexpr int NT = LuRegNT<T>::v;
+    if (n - j <= NT) {
+      hipLaunchKernelGGL((lu_panel_reg_kernel<T, NT>), dim3(1), dim3(NT), 0, st, A.p, A.ld, n, j, nb, ipiv);
+    } else {
+      int threads = 1024;
+      if (n - j <= 256) threads = 256;
+      else if (n - j <= 512) threads = 512;
+      hipLaunchKernelGGL(lu_panel_kernel<T>, dim3(1), dim3(threads), 0, st, A.p, A.ld, n, j, nb, ipiv);
+    }
     hipLaunchKernelGGL(laswp_kernel<T>, dim3((n + 127) / 128), dim3(128), 0, st, A.p, A.ld, n, j, nb,
                        (const int*)ipiv, j, j + nb);
     const int rest = n - j - nb;
