@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--plain-tol-accurate", type=int, default=9, help="TOL_POW of the accurate plain FP64 run")
     ap.add_argument("--headline", action="store_true",
                     help="only the refined-forms comparison on the 122,880-unknown sphere at 3.82 GHz")
+    ap.add_argument("--scatterers", type=int, default=0, metavar="S",
+                    help="only the batched-scatterers workload: S seeded ellipsoids, 8 right-hand sides each")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -126,6 +128,9 @@ def main():
         sys.exit(2)
     if args.headline:
         headline(args)
+        return
+    if args.scatterers:
+        scatterers(args)
         return
 
     import rcs_amd as R
@@ -313,6 +318,12 @@ def refined_forms(R, g, bb, b, forms, op_ref, op_check, log, net, sols=None):
         for name, v in out.items():
             ok = v["relres_gmres"] <= 1e-6 and base["relres_gmres"] <= 1e-6
             v["speedup_vs_plain_fp64_refined"] = base["total_s"] / v["total_s"] if ok else None
+    if sols is not None and "plain_fp64_refined" in sols:  # monostatic-sweep RCS against the base form
+        th = np.linspace(0, np.pi, 181)
+        ph = np.zeros_like(th)
+        r0 = 10 * np.log10(g.rcs(sols["plain_fp64_refined"], th, ph))
+        for name, x in sols.items():
+            out[name]["rcs_max_abs_db_diff_vs_plain_refined"] = float(np.max(np.abs(10 * np.log10(g.rcs(x, th, ph)) - r0)))
     return out
 
 
@@ -346,7 +357,7 @@ def accuracy_and_refined(R, g, bb, b, x_plain, args, log, net):
     res["refinement_operator"] = {"tol_pow": 8, "assembly_s": time.perf_counter() - t0}
     forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
              (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, net is not None)]
-    res["refined"] = refined_forms(R, g, bb, b, forms, opm, op, log, net)
+    res["refined"] = refined_forms(R, g, bb, b, forms, opm, op, log, net, {})
     del opm, op
     return res
 
@@ -390,6 +401,52 @@ def headline(args):
                       "refinement_operator": {"tol_pow": 8, "assembly_s": t_opm},
                       "check_operator": {"tol_pow": 10, "assembly_s": t_op},
                       "rank_predictor_training": train, "refined": ref}))
+
+
+def scatterers(args):
+    """Many independent scatterers: seeded ellipsoids with semi-axes in [0.05, 0.3] m meshed at
+    subdivision 3-5, EFIE at 1 GHz, each factorized in complex64 at TOL_POW args.mixed_tol and
+    refined by flexible GMRES against its own FP64 TOL_POW 8 operator to 1e-6 for 8 plane waves
+    at seeded incidence angles, one after another (concurrent lanes are switched off). Prints
+    one progress line per scatterer."""
+    import rcs_amd as R
+    from rcs_amd import mesh as M
+
+    def log(msg):
+        print(f"[bench scatterers] {msg}", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(2026)
+    kap = 2 * np.pi * 1e9 / 3.0e8
+    res = []
+    t0 = time.perf_counter()
+    for i in range(args.scatterers):
+        axes = tuple(float(a) for a in rng.uniform(0.05, 0.3, size=3))
+        level = int(rng.integers(3, 6))
+        th = rng.uniform(0, np.pi, size=8)
+        ph = rng.uniform(0, 2 * np.pi, size=8)
+        nodes, tris = M.ellipsoid(level, axes)
+        g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+        bb = M.bounding_box(g.locations())
+        k = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+        pol = np.stack([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)], 1)
+        B = np.column_stack([g.rhs(tuple(pol[c]), tuple(kap * k[c])) for c in range(8)])
+        ts = time.perf_counter()
+        op = R.Operator(g, bb, 64, 8)
+        s = R.AIFMM(g, bb, 64, args.mixed_tol, args.mixed_tol, 0, R.C64)
+        s.factorize()
+        X, it, rr = s.refine(op, B, tol=1e-6, maxit=60, restart=30)
+        del s, op
+        res.append((g.N, it, rr))
+        log(f"scatterer {i + 1}/{args.scatterers}: N={g.N}, {time.perf_counter() - ts:.2f} s, "
+            f"{it} GMRES iterations, relres {rr:.1e}")
+    wall = time.perf_counter() - t0
+    ns = [r[0] for r in res]
+    print(json.dumps({
+        "workload": f"{len(res)} seeded ellipsoids (semi-axes in [0.05, 0.3] m, subdivision 3-5, "
+                    f"N = {min(ns)}..{max(ns)}, total {sum(ns)}), EFIE 1 GHz, complex64 TOL {args.mixed_tol} + "
+                    "FP64 TOL 8 operator + flexible GMRES to 1e-6, 8 plane waves each, one after another",
+        "wall_s": wall, "scatterers_per_s": len(res) / wall, "rhs_per_s": 8 * len(res) / wall,
+        "max_relres": max(r[2] for r in res), "max_gmres_iterations": max(r[1] for r in res)}))
 
 
 if __name__ == "__main__":

@@ -128,6 +128,16 @@ int rcs_set_rank_model(rcs_mlp_t m);
 /* ACA steps launched between host checks once the predicted pass is over (default 1, the
  * reference's one step per check). Skeletons do not depend on it. */
 int rcs_set_aca_burst(int steps);
+/* Selects the library context ("lane": HIP stream, HBM pool, staging buffer, profiler) of the
+ * calling host thread; lane 0 is the default. Currently only lane 0 is accepted (other lanes
+ * return an error until concurrent lanes have been verified on the GPU). Meshes, solvers and operators keep the lane
+ * they were created in, and calls on them run on its stream, so independent problems created
+ * in different lanes from different host threads run concurrently on the GPU (independent
+ * scatterers). A solver and the operator refining it must share a lane; the objects of one
+ * lane are used from one host thread at a time (a rank model set with rcs_set_rank_model
+ * runs its predictions in its own lane). No counterpart in the reference (its solver is
+ * single-threaded per process). */
+int rcs_set_lane(int lane);
 /* Rank log of the assembly: 14 floats per ACA problem (8 predictor features, rank found,
  * rank predicted or 0, level, 0 incoming / 1 outgoing, short and long side lengths). Writes
  * up to cap_rows rows (out may be NULL) and returns the number of rows. */

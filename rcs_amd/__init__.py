@@ -69,6 +69,7 @@ def lib():
         L.rcs_mlp_train.argtypes = [vp, fp, fp, C.c_int, C.c_int, C.c_float, dp]
         L.rcs_set_rank_model.argtypes = [vp]
         L.rcs_set_aca_burst.argtypes = [C.c_int]
+        L.rcs_set_lane.argtypes = [C.c_int]
         L.rcs_solver_rank_log.argtypes = [vp, fp, C.c_int]
         L.rcs_solver_aca_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
         _LIB = L
@@ -82,7 +83,7 @@ EXPORTED = [
     "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
     "rcs_operator_create", "rcs_operator_destroy", "rcs_operator_apply", "rcs_solver_refine",
     "rcs_mlp_create", "rcs_mlp_destroy", "rcs_mlp_get", "rcs_mlp_set", "rcs_mlp_forward", "rcs_mlp_gradient",
-    "rcs_mlp_train", "rcs_set_rank_model", "rcs_set_aca_burst", "rcs_solver_rank_log", "rcs_solver_aca_stats",
+    "rcs_mlp_train", "rcs_set_rank_model", "rcs_set_aca_burst", "rcs_set_lane", "rcs_solver_rank_log", "rcs_solver_aca_stats",
 ]
 
 PROFILE_TAGS = ["schur", "getrf", "getrs", "entries", "aca", "rrqr", "update"]
@@ -380,6 +381,13 @@ class RankMLP:
 def set_rank_model(model):
     """Use `model` (a RankMLP, or None) for the ACA of solvers created afterwards."""
     _check(lib().rcs_set_rank_model(model.h if model is not None else None))
+
+
+def set_lane(lane):
+    """Library context of the calling thread (rcs_set_lane): objects created afterwards in this
+    thread run on that lane's stream, so independent problems in different lanes and threads
+    run concurrently on the GPU."""
+    _check(lib().rcs_set_lane(int(lane)))
 
 
 def set_aca_burst(steps):

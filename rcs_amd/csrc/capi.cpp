@@ -37,18 +37,24 @@ int g_aca_burst = 1;
 
 namespace {
 thread_local std::string g_err;
-std::unique_ptr<Ctx> g_ctx;
+// Library contexts (stream, HBM pool, staging ring, profiler), one per lane; a host thread
+// works in the lane it selected (0 by default). Objects keep the context they were created
+// in, and calls on an object run on that context's stream.
+constexpr int MAX_LANES = 64;
+std::vector<std::unique_ptr<Ctx>> g_ctxs;
 std::mutex g_mu;
+thread_local int g_lane = 0;
 
 Ctx& ctx() {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_ctx) {
+  if ((int)g_ctxs.size() <= g_lane) g_ctxs.resize(g_lane + 1);
+  if (!g_ctxs[g_lane]) {
     int n = 0;
     HIPCHK(hipGetDeviceCount(&n));
     if (n <= 0) throw std::runtime_error("no HIP device");
-    g_ctx.reset(new Ctx());
+    g_ctxs[g_lane].reset(new Ctx());
   }
-  return *g_ctx;
+  return *g_ctxs[g_lane];
 }
 int fail(const std::exception& e) {
   g_err = e.what();
@@ -206,6 +212,16 @@ int rcs_init(void) {
   CATCH
 }
 const char* rcs_last_error(void) { return g_err.c_str(); }
+int rcs_set_lane(int lane) {
+  TRY if (lane < 0 || lane >= MAX_LANES) throw std::runtime_error("lane out of range [0, 64)");
+  // Switched off: a run of 64 scatterers in 8 lanes went quiet past its time limit and left
+  // its process inside the GPU driver when it was stopped; until a multi-lane run has been
+  // verified, every thread works in lane 0.
+  if (lane != 0) throw std::runtime_error("concurrent lanes are switched off in this build (lane 0 only)");
+  g_lane = lane;
+  return 0;
+  CATCH
+}
 
 int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
                     int kernel_kind, double freq, rcs_mesh_t* out) {
@@ -470,7 +486,7 @@ int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol
 void rcs_operator_destroy(rcs_operator_t o) { delete o; }
 int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
   TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
-  Ctx& cx = ctx();
+  Ctx& cx = o->s->context();
   const size_t bytes = sizeof(cpx<double>) * (size_t)o->s->size() * nrhs;
   void* dx = cx.pool.alloc(bytes);
   void* dy = cx.pool.alloc(bytes);
@@ -486,7 +502,8 @@ int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
 int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int nrhs, double* x, double tol, int maxit,
                       int restart, int* iters, double* relres) {
   TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
-  Ctx& cx = ctx();
+  Ctx& cx = s->s->context();
+  if (&o->s->context() != &cx) throw std::runtime_error("refine: operator and solver were created in different lanes");
   const size_t bytes = sizeof(cpx<double>) * (size_t)s->s->size() * nrhs;
   cpx<double>* db = (cpx<double>*)cx.pool.alloc(bytes);
   cpx<double>* dx = (cpx<double>*)cx.pool.alloc(bytes);

@@ -2,6 +2,7 @@
 // launch parameter arrays (pinned host memory mirrored into HBM in stream order).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -28,13 +29,18 @@ class DevPool {
       in_use_ += sz;
       return p;
     }
-    // blocks cached here are HBM the HIP runtime cannot use: past the limit they go back first
+    // blocks cached here are HBM the HIP runtime cannot use: past the limit (all pools of the
+    // process together) this pool's cached blocks go back first
     if (limit_ == 0) {
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
       limit_ = tot / 100 * 85;
     }
-    if (total_ + bytes > limit_) trim_to(limit_ > bytes ? limit_ - bytes : 0);
+    if (all_pools_total() + bytes > limit_) {
+      const size_t others = all_pools_total() - total_;
+      const size_t room = limit_ > others + bytes ? limit_ - others - bytes : 0;
+      trim_to(room);
+    }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -44,6 +50,7 @@ class DevPool {
     }
     used_[p] = bytes;
     total_ += bytes;
+    g_total().fetch_add(bytes);
     in_use_ += bytes;
     peak_ = std::max(peak_, in_use_);
     return p;
@@ -57,7 +64,7 @@ class DevPool {
     used_.erase(it);
   }
   void trim() {
-    for (auto& kv : free_) { (void)hipFree(kv.second); total_ -= kv.first; }
+    for (auto& kv : free_) { (void)hipFree(kv.second); total_ -= kv.first; g_total().fetch_sub(kv.first); }
     free_.clear();
   }
   // Return cached blocks, largest first, until at most `target` bytes are held.
@@ -66,6 +73,7 @@ class DevPool {
       auto it = std::prev(free_.end());
       (void)hipFree(it->second);
       total_ -= it->first;
+      g_total().fetch_sub(it->first);
       free_.erase(it);
     }
   }
@@ -73,8 +81,10 @@ class DevPool {
     trim();
     for (auto& kv : used_) (void)hipFree(kv.first);
     used_.clear();
+    g_total().fetch_sub(total_);
     total_ = in_use_ = 0;
   }
+  static size_t all_pools_total() { return g_total().load(); }
   size_t in_use() const { return in_use_; }
   size_t peak() const { return peak_; }
   void reset_peak() { peak_ = in_use_; }
@@ -85,6 +95,10 @@ class DevPool {
   std::map<void*, size_t> used_;
   size_t total_ = 0, in_use_ = 0, peak_ = 0;
   size_t limit_ = 0;  // 85% of the device memory, set at the first allocation
+  static std::atomic<size_t>& g_total() {  // bytes held by every pool of the process
+    static std::atomic<size_t> t{0};
+    return t;
+  }
   bool no_reuse_ = std::getenv("RCS_DEBUG_NO_REUSE") != nullptr;  // debugging: never recycle blocks
 };
 
@@ -167,6 +181,10 @@ struct Ctx {
   double* d_scalars = nullptr;  // scratch for reductions read back by the host
   double* h_scalars = nullptr;
   size_t n_scalars = 0;
+  // per-tile arrival counters of split-inner-index GEMMs (zero between launches: the last
+  // workgroup of a tile resets its counter)
+  static constexpr int n_tile_counters = 256;
+  unsigned* d_tile_counters = nullptr;
 
   explicit Ctx(size_t stage_bytes = size_t(64) << 20, size_t nscal = 1 << 16) {
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -176,6 +194,8 @@ struct Ctx {
     n_scalars = nscal;
     HIPCHK(hipMalloc((void**)&d_scalars, sizeof(double) * nscal));
     HIPCHK(hipHostMalloc((void**)&h_scalars, sizeof(double) * nscal, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&d_tile_counters, sizeof(unsigned) * n_tile_counters));
+    HIPCHK(hipMemset(d_tile_counters, 0, sizeof(unsigned) * n_tile_counters));
   }
   ~Ctx() {
     if (st) (void)hipStreamSynchronize(st);
@@ -184,6 +204,7 @@ struct Ctx {
     if (d_stage) (void)hipFree(d_stage);
     if (d_scalars) (void)hipFree(d_scalars);
     if (h_scalars) (void)hipHostFree(h_scalars);
+    if (d_tile_counters) (void)hipFree(d_tile_counters);
     if (st) (void)hipStreamDestroy(st);
   }
   Ctx(const Ctx&) = delete;

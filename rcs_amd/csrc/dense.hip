@@ -160,10 +160,12 @@ __global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs<T> a) {
 }
 
 // Split inner dimension for groups with few output tiles: workgroup (tile, part) sums the
-// products over its slice of the inner index into part's 64x64 slot of `partial`; the
-// reduce kernel adds the parts of a tile in a fixed order and applies alpha and beta.
+// products over its slice of the inner index into part's 64x64 slot of `partial`, then
+// counts its arrival on the tile's counter; the workgroup that arrives last adds the parts
+// of the tile in index order (the same sum whichever arrives last), applies alpha and beta,
+// and resets the counter for the next launch.
 template <typename T>
-__global__ void __launch_bounds__(256) gemm_splitk_kernel(GemmArgs<T> a, int parts, T* partial) {
+__global__ void __launch_bounds__(256) gemm_splitk_kernel(GemmArgs<T> a, int parts, T* partial, unsigned* counters) {
   const int tile = blockIdx.x / parts, part = blockIdx.x % parts;
   const GemmOut<T> out = a.outs[find_out(a.outs, a.nouts, tile)];
   int ktot = 0;
@@ -183,27 +185,26 @@ __global__ void __launch_bounds__(256) gemm_splitk_kernel(GemmArgs<T> a, int par
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) slot[(ty + 16 * i) + (tx + 16 * j) * GT] = acc[i][j];
-}
-template <typename T>
-__global__ void __launch_bounds__(256) gemm_splitk_reduce_kernel(GemmArgs<T> a, int parts, const T* partial) {
-  const int tile = blockIdx.x;
-  const GemmOut<T> out = a.outs[find_out(a.outs, a.nouts, tile)];
-  const int t = tile - out.tile0, tm = (out.m + GT - 1) / GT;
-  const int m0 = (t % tm) * GT, n0 = (t / tm) * GT;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
+  __threadfence();  // this part is visible device-wide before its arrival is counted
+  __shared__ unsigned last;
+  __syncthreads();
+  if (tid == 0) last = (atomicAdd(&counters[tile], 1u) == (unsigned)parts - 1u) ? 1u : 0u;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = T(0, 0);
   for (int p = 0; p < parts; ++p) {
-    const T* slot = partial + ((size_t)tile * parts + p) * GT * GT;
+    const T* sp = partial + ((size_t)tile * parts + p) * GT * GT;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + slot[(ty + 16 * i) + (tx + 16 * j) * GT];
+      for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + sp[(ty + 16 * i) + (tx + 16 * j) * GT];
   }
   gemm_store(out, m0, n0, acc);
+  if (tid == 0) counters[tile] = 0u;
 }
 
 // Narrow products (at most NARROW_N output columns, op(A) = A, op(B) = B), as in solves with
@@ -345,6 +346,7 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
         kmax = std::max(kmax, kt);
       }
       parts = std::max(1, std::min((2 * SPLITK_TILES + tiles - 1) / tiles, kmax / SPLITK_MIN_K));
+      static_assert(SPLITK_TILES <= Ctx::n_tile_counters, "split GEMM tiles exceed the counter array");
     }
   } else {
     d_outs = (const GemmOut<T>*)cx.push(outs.data(), sizeof(GemmOut<T>) * outs.size());
@@ -354,9 +356,8 @@ void gemm_grouped(std::vector<GemmOp<T>>& ops, Ctx& cx, int role, int prof_tag) 
   // the profiled interval holds the kernel alone (the parameter copies come before it)
   const int h = prof_tag >= 0 ? cx.prof.begin(cx.st) : -1;
   if (parts > 1) {
-    hipLaunchKernelGGL((gemm_splitk_kernel<T>), dim3(tiles * parts), dim3(256), 0, cx.st, args, parts, partial);
-    hipLaunchKernelGGL((gemm_splitk_reduce_kernel<T>), dim3(tiles), dim3(256), 0, cx.st, args, parts,
-                       (const T*)partial);
+    hipLaunchKernelGGL((gemm_splitk_kernel<T>), dim3(tiles * parts), dim3(256), 0, cx.st, args, parts, partial,
+                       cx.d_tile_counters);
   } else if (small) {
     hipLaunchKernelGGL((gemm_small_kernel<T, 0>), dim3(tiles), dim3(256), 0, cx.st, args);
   } else if (role == 1) {
@@ -658,10 +659,7 @@ void getrf(DMat<T> A, int* ipiv, Ctx& cx) {
 // Few right-hand sides of a system of at most GETRS_LDS unknowns: one workgroup per
 // column, the column gathered through the permutation into LDS and solved there.
 constexpr int GETRS_LDS = 1024;
-// Solve with the LU factors for a few right-hand sides (one workgroup per column), in blocks
-// of 64 unknowns: the block's diagonal part of L (or U) is staged in LDS by all threads, one
-// wavefront solves it (lane i owns unknown i, the solved value is broadcast by a shuffle),
-// then all threads update the remaining unknowns with the block's columns (coalesced reads).
+
 // x_r - sum_{k < nb} LU(r, j0 + k) x_{j0 + k}, with sixteen loads of the row issued together.
 template <typename T>
 __device__ __forceinline__ T getrs_update(const T* LU, int ld, int r, int j0, int nb, const T* xl) {
@@ -678,6 +676,10 @@ __device__ __forceinline__ T getrs_update(const T* LU, int ld, int r, int j0, in
   return s;
 }
 
+// Solve with the LU factors for a few right-hand sides (one workgroup per column), in blocks
+// of 64 unknowns: the block's diagonal part of L (or U) is staged in LDS by all threads, one
+// wavefront solves it (lane i owns unknown i, the solved value is broadcast by a shuffle),
+// then all threads update the remaining unknowns with the block's columns (coalesced reads).
 template <typename T>
 __global__ void __launch_bounds__(256) getrs_vec_kernel(const T* LU, int ld, int n, const int* perm, T* B, int ldb) {
   __shared__ T xl[GETRS_LDS];

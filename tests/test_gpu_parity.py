@@ -323,6 +323,28 @@ def test_refinement_reaches_dense_solution(prec, tp_factor):
     assert _rel(x, xd) < 1e-8 and _rel(x, xd) < 1e-2 * _rel(x0, xd)
 
 
+def test_refined_rcs_within_005_db_of_dense():
+    """The mixed-precision form's monostatic-sweep RCS (complex64 TOL 4 factorization refined
+    to a relative residual of 1e-6 against an FP64 TOL 10 operator) agrees with the RCS of the
+    oracle's dense LU solution within 0.05 dB at every angle (19 angles, theta in [0, 180])."""
+    nodes, tris = M.icosphere(3, 0.15)
+    g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+    o = O.Mesh(nodes, tris)
+    bb = M.bounding_box(g.locations())
+    b = g.rhs()
+    xd = o.dense_solve(O.EFIE, 1e9, b)
+    s = R.AIFMM(g, bb, 64, 4, 4, 0, R.C64)
+    s.factorize()
+    op = R.Operator(g, bb, 64, 10)
+    x, it, rr = s.refine(op, b, tol=1e-6, maxit=60, restart=30)
+    th = np.linspace(0, np.pi, 19)
+    ph = np.zeros_like(th)
+    db = lambda v: 10 * np.log10(v)
+    diff = np.abs(db(g.rcs(x, th, ph)) - db(g.rcs(xd, th, ph)))
+    print(f"{it} GMRES iterations, relres {rr:.1e}, max RCS difference {diff.max():.2e} dB")
+    assert rr <= 1e-6 and diff.max() < 0.05
+
+
 def test_rank_mlp_matches_cpu_oracle(rng):
     """HIP forward and backward of the rank MLP against the numpy oracle (same BF16 weights)."""
     from oracle import rank_mlp as Q
@@ -390,3 +412,41 @@ def test_rank_model_trains_and_keeps_skeletons():
           f"{np.median(rel):.2f}, miss rate {miss:.2f}; ACA host checks {st0['syncs']} -> {st1['syncs']}")
     assert st1["syncs"] < st0["syncs"]
     assert np.median(rel) < 0.5
+
+
+@pytest.mark.skip(reason="concurrent lanes are switched off: a 64-scatterer run in 8 lanes went quiet past its "
+                         "time limit and left the GPU box unresponsive; rcs_set_lane accepts lane 0 only")
+def test_lanes_solve_independent_problems_concurrently():
+    """Two problems solved at the same time from two host threads in two library lanes give
+    the same solutions as one after another in the default lane."""
+    import threading
+    probs = [(M.ellipsoid(3, (0.3, 0.06, 0.06)), 1e9), (M.icosphere(3, 0.15), 2e9)]
+
+    def solve(mesh, freq):
+        g = R.Mesh(mesh[0], mesh[1], R.EFIE, freq)
+        bb = M.bounding_box(g.locations())
+        s = R.AIFMM(g, bb, 64, 5, 5, 0, R.C128)
+        s.factorize()
+        op = R.Operator(g, bb, 64, 8)
+        x, it, rr = s.refine(op, g.rhs(), tol=1e-8, maxit=40, restart=30)
+        return s.solve(g.rhs()), x
+
+    ref = [solve(*p) for p in probs]
+    out = [None, None]
+    err = []
+
+    def worker(i):
+        try:
+            R.set_lane(i + 1)
+            out[i] = solve(*probs[i])
+        except Exception as e:
+            err.append(repr(e))
+
+    ths = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not err, err
+    for (x0, xr0), (x1, xr1) in zip(ref, out):
+        assert _rel(x1, x0) < 1e-12 and _rel(xr1, xr0) < 1e-10

@@ -59,3 +59,14 @@ def test_abi_error_string_without_gpu_work():
     import rcs_amd
     lib = rcs_amd.lib()
     assert isinstance(lib.rcs_last_error(), bytes)
+
+
+def test_only_lane_zero_is_accepted():
+    """Concurrent lanes are switched off: lane 0 is accepted, any other lane is an error with a
+    message (no device work is involved)."""
+    import rcs_amd
+    L = rcs_amd.lib()
+    assert L.rcs_set_lane(0) == 0
+    assert L.rcs_set_lane(1) < 0
+    assert b"lane 0 only" in L.rcs_last_error()
+    assert L.rcs_set_lane(64) < 0
