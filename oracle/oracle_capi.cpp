@@ -226,8 +226,6 @@ int orc_aifmm_factorize(void* h) {
   ORC_TRY
   OSolver* s = (OSolver*)h;
   s->S->A->ctr = Counters();
-  const char* e = std::getenv("RCS_DEBUG_STOP");
-  if (e) std::sscanf(e, "%d:%d", &s->S->A->stopLevel, &s->S->A->stopBox);
   double t0 = now();
   s->S->factorize();
   s->t_factor = now() - t0;

@@ -763,11 +763,9 @@ struct Tree {
         B(j, k).Eliminated = true;
         eliminate_x(j, k);
         eliminate_z(j, k);
-        if (j == stopLevel && k == stopBox) return;
       }
     }
   }
-  int stopLevel = -1, stopBox = -1;  // debugging: stop right after eliminating (stopLevel, stopBox)
 
   // ---------------- solve (T:670-678, 3506-3535, 3613-3653, 4115-4148, 4749-4907) ----------------
   void assign_Leaf_rhs(const std::vector<cd>& b) {

@@ -1294,7 +1294,6 @@ class Solver : public SolverBase {
 
   void factorize() override {  // T:4599-4747
     double t0 = wall();
-    read_debug_env();
     for (int j = nLevels; j >= startLevel; --j) {
       if (j != nLevels) {
         initialise_P2P_NonLeafLevel(j);
@@ -1331,10 +1330,6 @@ class Solver : public SolverBase {
         if (k % 16 == 0) trace("eliminate level %d box %d", j, k);
         eliminate_x(j, k);
         eliminate_z(j, k);
-        if (j == dbg_level && k == dbg_box) {  // debugging aid: stop after this box
-          cx.sync();
-          return;
-        }
       }
     }
     trace("base level: %d boxes", nBoxes[startLevel]);
@@ -1749,8 +1744,6 @@ class Solver : public SolverBase {
   DMat<T> luBase, invBase;
   int* ipBase = nullptr;
   int NMb = 0, NLb = 0;
-  // RCS_DEBUG_STOP="level:box" stops the factorization after that box (inspection only)
-  int dbg_level = -1, dbg_box = -1;
   // RCS_VERBOSE=1 prints one line per phase to stderr (flushed)
   bool verbose = std::getenv("RCS_VERBOSE") != nullptr;
   // RCS_FULL_RRQR=1 runs the full partitioned RRQR even when the leading block is orthonormal
@@ -1762,12 +1755,6 @@ class Solver : public SolverBase {
     std::fprintf(stderr, fmt, a, b);
     std::fprintf(stderr, " (pool %.2f GiB)\n", cx.pool.in_use() / 1073741824.0);
     std::fflush(stderr);
-  }
-
- public:
-  void read_debug_env() {
-    const char* e = std::getenv("RCS_DEBUG_STOP");
-    if (e) std::sscanf(e, "%d:%d", &dbg_level, &dbg_box);
   }
 };
 

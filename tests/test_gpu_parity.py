@@ -19,10 +19,10 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("opA,opB", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200),
-                                   (300, 1, 700), (65, 8, 129), (100, 100, 900)])
+                                   (300, 1, 700), (65, 8, 129), (100, 100, 900), (700, 650, 301)])
 def test_gemm(rng, opA, opB, m, n, k):
-    """Covers the tile kernel, the split inner index (few tiles, long k) and the narrow
-    kernel (n <= 8, no transposes)."""
+    """Covers the 64 x 64 tile kernel (few and many tiles), the split inner index (few tiles,
+    long k) and the narrow kernel (n <= 8, no transposes)."""
     A = rng.normal(size=(m, k) if opA == 0 else (k, m)) + 1j * rng.normal(size=(m, k) if opA == 0 else (k, m))
     B = rng.normal(size=(k, n) if opB == 0 else (n, k)) + 1j * rng.normal(size=(k, n) if opB == 0 else (n, k))
     ref = (A if opA == 0 else A.conj().T) @ (B if opB == 0 else B.conj().T)
