@@ -123,7 +123,7 @@ int rcs_mlp_gradient(rcs_mlp_t m, const float* X, const float* t, int n, float* 
 int rcs_mlp_train(rcs_mlp_t m, const float* X, const float* t, int n, int iters, float lr, double* loss);
 /* Solvers created after this call predict every ACA rank with m (NULL: none) to size the
  * ACA buffers and launch that many steps before the first host check; the adaptive stop
- * stays the judge (a box keeps iterating past a low prediction). */
+ * still decides where each ACA ends (a box keeps iterating past a low prediction). */
 int rcs_set_rank_model(rcs_mlp_t m);
 /* ACA steps launched between host checks once the predicted pass is over (default 1, the
  * reference's one step per check). Skeletons do not depend on it. */

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only (see oracle_core.hpp).
 //
 // Plain C entry points over the FP64 restatement, loaded by the tests (ctypes) as the
-// checker for the HIP library and timed by bench.py as the CPU baseline. Complex
+// reference answer for the HIP library and timed by bench.py as the CPU baseline. Complex
 // arrays are interleaved (re, im) doubles, matrices column-major.
 #include <chrono>
 #include <cstdio>

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only.
 //
 // CPU FP64 restatement of the vaishna77/RCS AIFMM factor-and-solve path. It is the
-// checker for the HIP implementation in rcs_amd/: only tests/, __graft_entry__.smoke()
+// reference answer for the HIP implementation in rcs_amd/: only tests/, __graft_entry__.smoke()
 // and bench.py's cpu_baseline leg may load it, never the product path.
 //
 // Dense types and the Eigen operations the reference relies on (column-major complex

@@ -1,7 +1,7 @@
 """ORACLE — test infrastructure only.
 
 ctypes view of liboracle.so, the FP64 CPU restatement of the reference path. Used by
-tests/ as the checker, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+tests/ for the expected results, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
 The product path (rcs_amd/) never imports this module.
 """
 from __future__ import annotations

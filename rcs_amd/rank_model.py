@@ -4,8 +4,8 @@ Every assembly logs, per ACA problem (a box's incoming or outgoing skeleton at o
 the predictor's 8 features and the rank the adaptive ACA found (AIFMM.rank_log()). A model
 is trained on such logs (target log(1 + rank)) with the library's own HIP kernels and then
 handed to the library with set_rank_model(): solvers created afterwards size their ACA
-buffers and launch their first ACA steps from the predicted ranks, with the adaptive stop
-as the judge.
+buffers and launch their first ACA steps from the predicted ranks; the adaptive stop
+still decides where each ACA ends.
 """
 import numpy as np
 
