@@ -221,12 +221,26 @@ def main():
     own_log = s.rank_log()
     del s
     net = None
+    # the sections below add figures to the line; a failure in one is reported in the line
+    # instead of losing the benchmark's own measurement
     if not args.no_learned:
-        net, out["learned_rank"] = learned_rank(R, M, own_log, log)
+        try:
+            net, out["learned_rank"] = learned_rank(R, M, own_log, log)
+        except Exception as e:
+            out["learned_rank"] = {"error": repr(e)}
+            log(f"learned rank section failed: {e!r}")
     if not args.no_mixed:
-        out.update(accuracy_and_refined(R, g, bb, b, x, args, log, net))
+        try:
+            out.update(accuracy_and_refined(R, g, bb, b, x, args, log, net))
+        except Exception as e:
+            out["refined_error"] = repr(e)
+            log(f"accuracy / refined-forms section failed: {e!r}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": repr(e)}
+            log(f"CPU baseline failed: {e!r}")
     print(json.dumps(out))
 
 
