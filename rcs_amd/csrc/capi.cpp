@@ -217,7 +217,7 @@ int rcs_set_lane(int lane) {
   // Switched off: a run of 64 scatterers in 8 lanes went quiet past its time limit and left
   // its process inside the GPU driver when it was stopped; until a multi-lane run has been
   // verified, every thread works in lane 0.
-  if (lane != 0) throw std::runtime_error("concurrent lanes are switched off in this build (lane 0 only)");
+  if (lane != 0) throw std::runtime_error("concurrent lanes are switched off (lane 0 only)");
   g_lane = lane;
   return 0;
   CATCH

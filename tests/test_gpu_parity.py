@@ -415,7 +415,7 @@ def test_rank_model_trains_and_keeps_skeletons():
 
 
 @pytest.mark.skip(reason="concurrent lanes are switched off: a 64-scatterer run in 8 lanes went quiet past its "
-                         "time limit and left the GPU box unresponsive; rcs_set_lane accepts lane 0 only")
+                         "time limit and left the machine unresponsive; rcs_set_lane accepts lane 0 only")
 def test_lanes_solve_independent_problems_concurrently():
     """Two problems solved at the same time from two host threads in two library lanes give
     the same solutions as one after another in the default lane."""
