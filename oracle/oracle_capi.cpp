@@ -34,6 +34,16 @@ Kernel* makeKernel(const OMesh* m, int kind, double freq) {
   }
   return new Laplace(&m->mesh);
 }
+
+// Entries served from a caller's column-major m x n matrix, so that the ACA can be run on
+// matrices whose cross approximation is known in closed form.
+struct DenseMatrix : Kernel {
+  const cd* a;
+  int m, n;
+  DenseMatrix(const cd* data, int rows, int cols) : a(data), m(rows), n(cols) {}
+  int size() const override { return std::max(m, n); }
+  cd entry(int i, int j) const override { return a[(size_t)i + (size_t)j * m]; }
+};
 }  // namespace
 
 #define ORC_TRY try {
@@ -129,6 +139,20 @@ int orc_aca(void* h, int kind, double freq, const int* rows, int nr, const int* 
   std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
   std::vector<int> r(rows, rows + nr), c(cols, cols + nc);
   ACAResult res = aca_only_nodes2(*K, tol, r, c);
+  *rank = res.rank;
+  for (int i = 0; i < res.rank; ++i) { row_bases[i] = res.row_bases[i]; col_bases[i] = res.col_bases[i]; }
+  return 0;
+  ORC_CATCH
+}
+
+// The same ACA over all rows and columns of a column-major m x n complex matrix A.
+int orc_aca_dense(const double* A, int m, int n, double tol, int* rank, int* row_bases, int* col_bases) {
+  ORC_TRY
+  DenseMatrix K(reinterpret_cast<const cd*>(A), m, n);
+  std::vector<int> r(m), c(n);
+  for (int i = 0; i < m; ++i) r[i] = i;
+  for (int j = 0; j < n; ++j) c[j] = j;
+  ACAResult res = aca_only_nodes2(K, tol, r, c);
   *rank = res.rank;
   for (int i = 0; i < res.rank; ++i) { row_bases[i] = res.row_bases[i]; col_bases[i] = res.col_bases[i]; }
   return 0;

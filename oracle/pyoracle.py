@@ -37,6 +37,7 @@ def lib():
         L.orc_rhs.argtypes = [vp, C.c_double, dp, dp, dp]
         L.orc_rcs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int, dp]
         L.orc_aca.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, C.c_double, ip, ip, ip]
+        L.orc_aca_dense.argtypes = [dp, C.c_int, C.c_int, C.c_double, ip, ip, ip]
         L.orc_rrqr.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                dp, dp, ip, ip, dp]
         L.orc_lu_solve.argtypes = [dp, C.c_int, dp, C.c_int, dp]
@@ -155,6 +156,20 @@ class Mesh:
         _check(lib().orc_matvec(self.h, kind, freq, x.ctypes.data_as(C.POINTER(C.c_double)),
                                 y.ctypes.data_as(C.POINTER(C.c_double))))
         return y
+
+
+def aca_dense(A, tol):
+    """ACA_only_nodes2 over every row and column of the complex matrix A: (rank, pivot rows,
+    pivot columns)."""
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m, n = A.shape
+    mn = min(m, n)
+    rk = np.zeros(1, dtype=np.int32)
+    rb = np.zeros(mn, dtype=np.int32)
+    cb = np.zeros(mn, dtype=np.int32)
+    _check(lib().orc_aca_dense(A.ctypes.data_as(C.POINTER(C.c_double)), m, n, tol, _ip(rk), _ip(rb), _ip(cb)))
+    r = int(rk[0])
+    return r, rb[:r].copy(), cb[:r].copy()
 
 
 def rrqr(A, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank):

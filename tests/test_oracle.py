@@ -145,6 +145,67 @@ def test_aca_exact_rank():
     assert r <= 2
 
 
+def test_aca_stop_rule_diagonal_known_answer():
+    """The stopping rule of src/ACA.cpp:702-756 worked by hand on a diagonal matrix.
+
+    Pivots go down the diagonal; every new pair is u = d_k e_k, v = e_k, orthogonal to the
+    earlier ones. The running |S| starts at 0 after the first pair (so d_0 never enters it) and
+    each update adds |u|^2|v|^2 plus twice the sum over all pairs including the new one, i.e.
+    3 d_k^2. With d = (1000, 1, 0.5, 0.4, 0.3) and tol = 0.3: after pair 1, 1 > 0.3*sqrt(3);
+    after pair 2, 0.5 <= 0.3*sqrt(3*1.25) = 0.581, so the rank is 3. Counting d_0 would stop
+    at rank 2; leaving out the newest pair's own term would run on to rank 5."""
+    d = np.array([1000.0, 1.0, 0.5, 0.4, 0.3])
+    K = np.zeros((5, 6), dtype=np.complex128)
+    K[np.arange(5), np.arange(5)] = d
+    for A, transposed in ((K, False), (K.T, True)):  # row-first (N1 < N2) and column-first branches
+        r, rb, cb = O.aca_dense(A, 0.3)
+        assert r == 3, (transposed, r)
+        assert list(rb) == [0, 1, 2] and list(cb) == [0, 1, 2]
+    # just above and below the hand-computed threshold 0.5 / sqrt(3.75) for the third pair
+    thr = 0.5 / np.sqrt(3.75)
+    assert O.aca_dense(K, thr * 1.001)[0] == 3
+    assert O.aca_dense(K, thr * 0.999)[0] == 4
+    assert O.aca_dense(K, 0.0)[0] == 5  # tol 0 runs to min(N1, N2)
+
+
+def test_aca_stop_rule_cross_term_known_answer():
+    """The cross terms 2|<u_l,u>||<v_l,v>| of the |S| update (src/ACA.cpp:751-753), by hand.
+
+    K = [[10,5,0,0],[2,4,0,0],[1,0.5,0,7]]: pivot (0,0) gives v_0 = (1,.5,0,0), u_0 = (10,2,1);
+    the next row is 1, its residual (0,3,0,0) pivots at column 1, so v_1 = e_1 and the residual
+    column is u_1 = (0,3,0). <u_0,u_1> = 6, <v_0,v_1> = 1/2, hence
+    |S|^2 = 9 + 2(6*1/2 + 9*1) = 33 and the ACA stops at rank 2 exactly when tol >= 3/sqrt(33).
+    Without the cross term the threshold would be 3/sqrt(27). A global phase changes nothing."""
+    K = np.array([[10, 5, 0, 0], [2, 4, 0, 0], [1, 0.5, 0, 7]], dtype=np.complex128)
+    thr = 3 / np.sqrt(33)
+    for A in (K, K * np.exp(0.7j)):
+        for B, transposed in ((A, False), (A.T, True)):
+            r, rb, cb = O.aca_dense(B, 0.55)  # between 3/sqrt(33) = 0.522 and 3/sqrt(27) = 0.577
+            assert r == 2, (transposed, r)
+            piv = (list(cb), list(rb)) if transposed else (list(rb), list(cb))
+            assert piv == ([0, 1], [0, 1])
+            assert O.aca_dense(B, thr * 1.001)[0] == 2
+            assert O.aca_dense(B, thr * 0.999)[0] == 3
+
+
+def test_aca_zero_rows_and_zero_matrix():
+    """src/ACA.cpp:669-684: the first pivot row is the first row with an entry above 1e-16;
+    an all-zero block has rank 0.
+
+    Row 0 holds 1e-17 everywhere, so the scan passes it and pivots at (1, 3). Row 0 stays in
+    the remaining set and, as u_0 = K[:, 3] = (1e-17, 2, 0), is the next pivot row; its
+    residual (1e-17 except 0 at column 3) pivots at column 0 (first maximum). Row 2 = e_1 is
+    last, at column 1. Every step is finite, and tol 1e-6 stops nothing before min(N1, N2)."""
+    K = np.zeros((3, 5), dtype=np.complex128)
+    assert O.aca_dense(K, 1e-6)[0] == 0
+    K[0, :] = 1e-17
+    K[1, 3] = 2.0
+    K[2, 1] = 1.0
+    r, rb, cb = O.aca_dense(K, 1e-6)
+    assert r == 3
+    assert list(rb) == [1, 0, 2] and list(cb) == [3, 0, 1]
+
+
 def test_rrqr_plain_cpqr_matches_lapack(rng):
     A = rng.normal(size=(40, 25)) + 1j * rng.normal(size=(40, 25))
     Q, R, perm, r, mp = O.rrqr(A, 14, 14, 0, 0.0, 1000)
