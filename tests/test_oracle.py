@@ -252,6 +252,57 @@ def test_rrqr_first_partition_early_stop_keeps_rank_minus_one():
     assert np.linalg.norm(q[:, :2] - Q @ (Q.conj().T @ q[:, :2])) < 1e-12
 
 
+def _orthogonal_columns(rng, m, s):
+    """U diag(s) with a random unitary U (m x len(s)), columns then shuffled: pivoted QR takes
+    the columns in descending s and |R_jj| = s_j exactly up to rounding."""
+    U = np.linalg.qr(rng.normal(size=(m, len(s))) + 1j * rng.normal(size=(m, len(s))))[0]
+    perm = rng.permutation(len(s))
+    return (U * np.asarray(s))[:, perm], perm
+
+
+def test_rrqr_pivot_thresholds_known_answer(rng):
+    """src/rrqr.cpp:147-155 worked by hand on columns of norms s along orthogonal directions.
+
+    Without a partition the kept rank is the number of leading pivots with
+    s_j / max(inputPivot, s_0) >= tol2, capped by fillInRank. s = (1, .3, .1, .02, .005):
+    tol2 = 1e-2 keeps 4; an input pivot of 3 makes the reference 3 and keeps 3 (ignoring it
+    would keep 4); 50 keeps only s_0 (.3/50 < 1e-2); fillInRank 2 keeps 2."""
+    s = [1.0, 0.3, 0.1, 0.02, 0.005]
+    A, perm = _orthogonal_columns(rng, 12, s)
+    order = list(np.argsort(perm)[:5])  # positions of s_0, s_1, ... in A
+    for inp, cap, want in ((0.0, 1000, 4), (3.0, 1000, 3), (0.0, 2, 2), (0.5, 1000, 4), (50.0, 1000, 1)):
+        Q, R, p, r, mp = O.rrqr(A, 2, 2, 0, inp, cap)
+        assert r == want, (inp, cap, r)
+        assert list(p[:r]) == order[:r]
+        assert np.allclose(np.abs(np.diag(R[:, :r])), s[:r], rtol=1e-12)
+        assert abs(mp - max(inp, 1.0)) < 1e-12 * max(inp, 1.0)  # the reported maxPivot
+
+
+def test_rrqr_second_partition_measures_against_its_own_first_pivot(rng):
+    """src/rrqr.cpp:111-118 and 146-151: the first partition's ratios use its first pivot, the
+    second partition's use max(inputPivot, its own first pivot), not the first partition's.
+
+    Partition columns of norms (1, .5), fill-in columns (.001, .0005), tol 1e-2 on both:
+    the first partition keeps 2; the second partition's first pivot .001 is its own reference,
+    so it keeps 2 more (rank 4). Measured against the first partition's 1 it would keep none.
+    An input pivot of 1 does make the reference 1 and the rank 2."""
+    m = 10
+    U = np.linalg.qr(rng.normal(size=(m, 4)) + 1j * rng.normal(size=(m, 4)))[0]
+    A = U * np.array([0.5, 1.0, 0.0005, 0.001])  # partition columns in ascending order on purpose
+    Q, R, p, r, mp = O.rrqr(A, 2, 2, 2, 0.0, 1000)
+    assert r == 4
+    assert list(p[:4]) == [1, 0, 3, 2]
+    assert abs(mp - 0.001) < 1e-15
+    Q, R, p, r, mp = O.rrqr(A, 2, 2, 2, 1.0, 1000)
+    assert r == 2 and list(p[:2]) == [1, 0]
+    # a first-partition stop: norms (1, .004) with tol1 1e-2 stops at j = 1 and keeps j - 1 = 0
+    # there; the second partition then restarts at position 0 over all columns with the
+    # partition's unused columns zeroed, and the fill-in (.001, .0005) is measured on its own
+    B = U * np.array([1.0, 0.004, 0.0005, 0.001])
+    Q, R, p, r, mp = O.rrqr(B, 2, 2, 2, 0.0, 1000)
+    assert r == 2
+
+
 def _aifmm_vs_dense(kind, freq, tols):
     """Elongated ellipsoid: the cubic octree puts ~80-130 unknowns in each non-empty leaf,
     so skeletons are genuinely low rank and the RRQR fill-in path runs."""
