@@ -60,6 +60,22 @@ int fail(const std::exception& e) {
   g_err = e.what();
   return -1;
 }
+// Argument checks, all on the host and before any device work: a bad argument becomes an
+// error code and a message, never a wild pointer or an out-of-range index in a kernel.
+void need(const void* p, const char* what) {
+  if (!p) throw std::runtime_error(std::string(what) + " is null");
+}
+void check_indices(const int* idx, int n, int N, const char* what) {
+  if (n > 0) need(idx, what);
+  for (int i = 0; i < n; ++i)
+    if (idx[i] < 0 || idx[i] >= N) throw std::runtime_error(std::string(what) + " index out of range");
+}
+void check_bbox(const double* bb) {
+  need(bb, "bbox");
+  for (int d = 0; d < 3; ++d)
+    if (!(bb[2 * d + 1] >= bb[2 * d])) throw std::runtime_error("bbox must list min <= max for x, y and z");
+  if (!(std::max({bb[1] - bb[0], bb[3] - bb[2], bb[5] - bb[4]}) > 0)) throw std::runtime_error("bbox is empty");
+}
 template <typename T>
 DMat<T> dalloc(Ctx& cx, int r, int c) {
   DMat<T> m;
@@ -227,6 +243,10 @@ int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tri
                     int kernel_kind, double freq, rcs_mesh_t* out) {
   TRY if (kernel_kind != KIND_EFIE && kernel_kind != KIND_LAPLACE) throw std::runtime_error("unknown kernel kind");
   if (!(freq > 0)) throw std::runtime_error("frequency must be positive");
+  if (n_nodes < 3 || n_tris < 1) throw std::runtime_error("a mesh needs at least 3 nodes and 1 triangle");
+  need(nodes, "nodes");
+  need(tris, "tris");
+  need(out, "out");
   std::unique_ptr<rcs_mesh_s> m(new rcs_mesh_s);
   m->g = std::make_shared<DeviceGeometry>();
   m->g->h.kind = kernel_kind;
@@ -240,14 +260,19 @@ int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tri
 void rcs_mesh_destroy(rcs_mesh_t m) { delete m; }
 int rcs_mesh_size(rcs_mesh_t m) { return m ? m->g->h.n_unknowns() : -1; }
 int rcs_mesh_locations(rcs_mesh_t m, double* xyz) {
-  TRY std::vector<double> v;
+  TRY need(m, "mesh");
+  need(xyz, "xyz");
+  std::vector<double> v;
   m->g->h.locations(v);
   std::memcpy(xyz, v.data(), sizeof(double) * v.size());
   return 0;
   CATCH
 }
 int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length) {
-  TRY const HostGeometry& h = m->g->h;
+  TRY need(m, "mesh");
+  need(out6, "out6");
+  need(length, "length");
+  const HostGeometry& h = m->g->h;
   for (size_t i = 0; i < h.n1.size(); ++i) {
     int* o = out6 + 6 * i;
     o[0] = h.n1[i]; o[1] = h.n2[i]; o[2] = h.tp[i]; o[3] = h.tm[i]; o[4] = h.fp[i]; o[5] = h.fm[i];
@@ -258,9 +283,11 @@ int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length) {
 }
 
 int rcs_entries(rcs_mesh_t m, int precision, const int* rows, int nr, const int* cols, int nc, double* out) {
-  TRY const int N = m->g->h.n_unknowns();
-  for (int i = 0; i < nr; ++i) if (rows[i] < 0 || rows[i] >= N) throw std::runtime_error("row index out of range");
-  for (int i = 0; i < nc; ++i) if (cols[i] < 0 || cols[i] >= N) throw std::runtime_error("column index out of range");
+  TRY need(m, "mesh");
+  const int N = m->g->h.n_unknowns();
+  check_indices(rows, nr, N, "row");
+  check_indices(cols, nc, N, "column");
+  if (nr > 0 && nc > 0) need(out, "out");
   if (precision == PREC_C128) entries_t<cpx<double>>(m, rows, nr, cols, nc, out);
   else if (precision == PREC_C64) entries_t<cpx<float>>(m, rows, nr, cols, nc, out);
   else throw std::runtime_error("unknown precision");
@@ -269,7 +296,11 @@ int rcs_entries(rcs_mesh_t m, int precision, const int* rows, int nr, const int*
 }
 
 int rcs_rhs(rcs_mesh_t m, const double* E, const double* kvec, double* out) {
-  TRY const HostGeometry& h = m->g->h;
+  TRY need(m, "mesh");
+  need(E, "E");
+  need(kvec, "kvec");
+  need(out, "out");
+  const HostGeometry& h = m->g->h;
   if (h.kind != KIND_EFIE) throw std::runtime_error("rcs_rhs needs an EFIE mesh");
   for (size_t i = 0; i < h.n1.size(); ++i) {
     int P = h.tp[i], M = h.tm[i];
@@ -288,7 +319,11 @@ int rcs_rhs(rcs_mesh_t m, const double* E, const double* kvec, double* out) {
 }
 
 int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double* phi, int n, double* out) {
-  TRY const HostGeometry& h = m->g->h;
+  TRY need(m, "mesh");
+  need(coef, "coef");
+  if (n < 0) throw std::runtime_error("n must not be negative");
+  if (n > 0) { need(theta, "theta"); need(phi, "phi"); need(out, "out"); }
+  const HostGeometry& h = m->g->h;
   if (h.kind != KIND_EFIE) throw std::runtime_error("rcs_rcs needs an EFIE mesh");
   const size_t F = h.v1.size();
   std::vector<double> J(6 * F, 0.0);  // (Jx, Jy, Jz) complex per patch
@@ -333,7 +368,14 @@ int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double*
 
 int rcs_aca(rcs_mesh_t m, int precision, double tol, const int* rows, int nr, const int* cols, int nc, int* rank,
             int* row_bases, int* col_bases) {
-  TRY if (nr <= 0 || nc <= 0) { *rank = 0; return 0; }
+  TRY need(m, "mesh");
+  need(rank, "rank");
+  if (nr <= 0 || nc <= 0) { *rank = 0; return 0; }
+  const int N = m->g->h.n_unknowns();
+  check_indices(rows, nr, N, "row");
+  check_indices(cols, nc, N, "column");
+  need(row_bases, "row_bases");
+  need(col_bases, "col_bases");
   if (precision == PREC_C128) aca_t<cpx<double>>(m, tol, rows, nr, cols, nc, rank, row_bases, col_bases);
   else if (precision == PREC_C64) aca_t<cpx<float>>(m, tol, rows, nr, cols, nc, rank, row_bases, col_bases);
   else throw std::runtime_error("unknown precision");
@@ -343,8 +385,13 @@ int rcs_aca(rcs_mesh_t m, int precision, double tol, const int* rows, int nr, co
 
 int rcs_rrqr(int precision, const double* A, int m, int n, int tol_pow1, int tol_pow2, int partition,
              double input_pivot, int fillin_rank, double* Q, int* rank, double* max_pivot) {
-  TRY if (partition < 0 || partition > n) throw std::runtime_error("partition out of range");
-  if (m <= 0 || n <= 0) { *rank = 0; *max_pivot = 0; return 0; }
+  TRY need(rank, "rank");
+  need(max_pivot, "max_pivot");
+  if (m < 0 || n < 0) throw std::runtime_error("m and n must not be negative");
+  if (partition < 0 || partition > n) throw std::runtime_error("partition out of range");
+  if (m == 0 || n == 0) { *rank = 0; *max_pivot = 0; return 0; }
+  need(A, "A");
+  need(Q, "Q");
   if (precision == PREC_C128) rrqr_t<cpx<double>>(A, m, n, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, Q, rank, max_pivot);
   else if (precision == PREC_C64) rrqr_t<cpx<float>>(A, m, n, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, Q, rank, max_pivot);
   else throw std::runtime_error("unknown precision");
@@ -353,7 +400,11 @@ int rcs_rrqr(int precision, const double* A, int m, int n, int tol_pow1, int tol
 }
 
 int rcs_lu_solve(int precision, const double* A, int n, const double* B, int nrhs, double* X) {
-  TRY if (n <= 0 || nrhs <= 0) return 0;
+  TRY if (n < 0 || nrhs < 0) throw std::runtime_error("n and nrhs must not be negative");
+  if (n == 0 || nrhs == 0) return 0;
+  need(A, "A");
+  need(B, "B");
+  need(X, "X");
   if (precision == PREC_C128) lu_t<cpx<double>>(A, n, B, nrhs, X);
   else if (precision == PREC_C64) lu_t<cpx<float>>(A, n, B, nrhs, X);
   else throw std::runtime_error("unknown precision");
@@ -363,8 +414,16 @@ int rcs_lu_solve(int precision, const double* A, int n, const double* B, int nrh
 
 int rcs_gemm(int precision, int opA, int opB, int m, int n, int k, const double* A, int lda, const double* B,
              int ldb, double* C) {
-  TRY if (m <= 0 || n <= 0) return 0;
+  TRY if (m < 0 || n < 0 || k < 0) throw std::runtime_error("m, n and k must not be negative");
   if ((opA != 0 && opA != 1) || (opB != 0 && opB != 1)) throw std::runtime_error("op must be 0 or 1");
+  if (m == 0 || n == 0) return 0;
+  if (k > 0) {  // with k = 0 neither operand is read
+    if (lda < (opA == OP_N ? m : k) || ldb < (opB == OP_N ? k : n))
+      throw std::runtime_error("leading dimension smaller than the stored rows");
+    need(A, "A");
+    need(B, "B");
+  }
+  need(C, "C");
   if (precision == PREC_C128) gemm_t<cpx<double>>(opA, opB, m, n, k, A, lda, B, ldb, C);
   else if (precision == PREC_C64) gemm_t<cpx<float>>(opA, opB, m, n, k, A, lda, B, ldb, C);
   else throw std::runtime_error("unknown precision");
@@ -375,6 +434,11 @@ int rcs_gemm(int precision, int opA, int opB, int m, int n, int k, const double*
 int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, int fillin_tol, int fillin_rank,
                       int precision, int variant, rcs_solver_t* out) {
   TRY if (leaf_size <= 0) throw std::runtime_error("leaf size must be positive");
+  if (variant != 0 && variant != 1) throw std::runtime_error("variant must be 0 or 1");
+  if (precision != PREC_C128 && precision != PREC_C64) throw std::runtime_error("unknown precision");
+  check_bbox(bbox);
+  need(m, "mesh");
+  need(out, "out");
   SolverParams p;
   for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];
   p.leafSize = leaf_size;
@@ -393,14 +457,16 @@ int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_p
   CATCH
 }
 int rcs_solver_rank_log(rcs_solver_t s, float* out, int cap_rows) {
-  TRY const std::vector<float>& v = s->s->rank_log();
+  TRY need(s, "solver");
+  const std::vector<float>& v = s->s->rank_log();
   const int rows = (int)(v.size() / RANK_LOG_COLS);
   if (out) std::memcpy(out, v.data(), sizeof(float) * RANK_LOG_COLS * std::min(rows, std::max(cap_rows, 0)));
   return rows;
   CATCH
 }
 int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, double* seconds) {
-  TRY SolverStats st = s->s->stats();
+  TRY need(s, "solver");
+  SolverStats st = s->s->stats();
   if (steps) *steps = st.aca_steps;
   if (syncs) *syncs = st.aca_syncs;
   if (seconds) *seconds = st.t_aca;
@@ -408,7 +474,8 @@ int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, dou
   CATCH
 }
 int rcs_mlp_create(unsigned seed, rcs_mlp_t* out) {
-  TRY std::unique_ptr<rcs_mlp_s> m(new rcs_mlp_s);
+  TRY need(out, "out");
+  std::unique_ptr<rcs_mlp_s> m(new rcs_mlp_s);
   m->m.reset(new RankMLP(ctx(), seed));
   *out = m.release();
   return 0;
@@ -419,28 +486,43 @@ void rcs_mlp_destroy(rcs_mlp_t m) {
   delete m;
 }
 int rcs_mlp_get(rcs_mlp_t m, float* params, float* mean, float* scale) {
-  TRY m->m->get(params, mean, scale);
+  TRY need(m, "model");
+  m->m->get(params, mean, scale);
   return 0;
   CATCH
 }
 int rcs_mlp_set(rcs_mlp_t m, const float* params, const float* mean, const float* scale) {
-  TRY m->m->set(params, mean, scale);
+  TRY need(m, "model");
+  need(params, "params");
+  need(mean, "mean");
+  need(scale, "scale");
+  m->m->set(params, mean, scale);
   return 0;
   CATCH
 }
 int rcs_mlp_forward(rcs_mlp_t m, const float* X, int n, float* y) {
-  TRY m->m->forward_host(X, n, y);
+  TRY need(m, "model");
+  if (n < 0) throw std::runtime_error("n must not be negative");
+  if (n > 0) { need(X, "X"); need(y, "y"); }
+  m->m->forward_host(X, n, y);
   return 0;
   CATCH
 }
 int rcs_mlp_gradient(rcs_mlp_t m, const float* X, const float* t, int n, float* grad, double* loss) {
-  TRY double l = m->m->gradient_host(X, t, n, grad);
+  TRY need(m, "model");
+  if (n < 0) throw std::runtime_error("n must not be negative");
+  if (n > 0) { need(X, "X"); need(t, "t"); }
+  need(grad, "grad");
+  double l = m->m->gradient_host(X, t, n, grad);
   if (loss) *loss = l;
   return 0;
   CATCH
 }
 int rcs_mlp_train(rcs_mlp_t m, const float* X, const float* t, int n, int iters, float lr, double* loss) {
-  TRY double l = m->m->train_host(X, t, n, iters, lr);
+  TRY need(m, "model");
+  if (n < 0 || iters < 0) throw std::runtime_error("n and iters must not be negative");
+  if (n > 0) { need(X, "X"); need(t, "t"); }
+  double l = m->m->train_host(X, t, n, iters, lr);
   if (loss) *loss = l;
   return 0;
   CATCH
@@ -458,18 +540,25 @@ int rcs_set_aca_burst(int steps) {
 }
 void rcs_solver_destroy(rcs_solver_t s) { delete s; }
 int rcs_solver_factorize(rcs_solver_t s) {
-  TRY s->s->factorize();
+  TRY need(s, "solver");
+  s->s->factorize();
   return 0;
   CATCH
 }
 int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, double* x_box) {
-  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  TRY need(s, "solver");
+  if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  need(rhs, "rhs");
+  need(x, "x");
   s->s->solve(rhs, nrhs, x, x_box);
   return 0;
   CATCH
 }
 int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, rcs_operator_t* out) {
   TRY if (leaf_size <= 0) throw std::runtime_error("leaf size must be positive");
+  check_bbox(bbox);
+  need(m, "mesh");
+  need(out, "out");
   SolverParams p;
   for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];
   p.leafSize = leaf_size;
@@ -485,7 +574,10 @@ int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol
 }
 void rcs_operator_destroy(rcs_operator_t o) { delete o; }
 int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
-  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  TRY need(o, "operator");
+  if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  need(x, "x");
+  need(y, "y");
   Ctx& cx = o->s->context();
   const size_t bytes = sizeof(cpx<double>) * (size_t)o->s->size() * nrhs;
   void* dx = cx.pool.alloc(bytes);
@@ -501,7 +593,13 @@ int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
 }
 int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int nrhs, double* x, double tol, int maxit,
                       int restart, int* iters, double* relres) {
-  TRY if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  TRY need(s, "solver");
+  need(o, "operator");
+  if (nrhs <= 0) throw std::runtime_error("nrhs must be positive");
+  if (maxit < 0) throw std::runtime_error("maxit must not be negative");
+  need(rhs, "rhs");
+  need(x, "x");
+  if (s->s->size() != o->s->size()) throw std::runtime_error("refine: operator and solver sizes differ");
   Ctx& cx = s->s->context();
   if (&o->s->context() != &cx) throw std::runtime_error("refine: operator and solver were created in different lanes");
   const size_t bytes = sizeof(cpx<double>) * (size_t)s->s->size() * nrhs;
@@ -519,7 +617,10 @@ int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int n
   CATCH
 }
 int rcs_solver_stats(rcs_solver_t s, int* info, double* vals) {
-  TRY SolverStats st = s->s->stats();
+  TRY need(s, "solver");
+  need(info, "info");
+  need(vals, "vals");
+  SolverStats st = s->s->stats();
   info[0] = st.nLevels; info[1] = st.startLevel; info[2] = (int)st.n_rrqr; info[3] = (int)st.n_compress_p2p;
   info[4] = (int)st.n_compress_p2l;
   vals[0] = st.t_assemble; vals[1] = st.t_factor; vals[2] = st.t_solve; vals[3] = st.gemm_flops;
@@ -529,12 +630,15 @@ int rcs_solver_stats(rcs_solver_t s, int* info, double* vals) {
   CATCH
 }
 int rcs_solver_block(rcs_solver_t s, int level, int box, int which, int key, int* shape, double* out, long long cap) {
-  TRY int r = 0, c = 0;
+  TRY need(s, "solver");
+  need(shape, "shape");
+  int r = 0, c = 0;
   std::vector<double> v;
   if (!s->s->block(level, box, which, key, r, c, v)) return 1;
   shape[0] = r;
   shape[1] = c;
   if ((long long)v.size() > 2 * cap) throw std::runtime_error("block buffer too small");
+  if (!v.empty()) need(out, "out");
   std::memcpy(out, v.data(), sizeof(double) * v.size());
   return 0;
   CATCH
@@ -552,6 +656,9 @@ int rcs_profile_reset(void) {
 }
 int rcs_profile_get(int tag, double* work, double* ms, long long* launches) {
   TRY if (tag < 0 || tag >= PROF_NTAGS) throw std::runtime_error("unknown profile tag");
+  need(work, "work");
+  need(ms, "ms");
+  need(launches, "launches");
   Ctx& cx = ctx();
   Profiler::Agg a = cx.prof.get(cx.st, tag);
   *work = a.work;
@@ -561,9 +668,11 @@ int rcs_profile_get(int tag, double* work, double* ms, long long* launches) {
   CATCH
 }
 int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap) {
-  TRY std::vector<int> v;
+  TRY need(s, "solver");
+  std::vector<int> v;
   s->s->ranks(level, v);
   if ((int)v.size() > cap) throw std::runtime_error("rank buffer too small");
+  if (!v.empty()) need(out, "out");
   std::memcpy(out, v.data(), sizeof(int) * v.size());
   return (int)v.size() / 3;
   CATCH

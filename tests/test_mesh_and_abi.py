@@ -70,3 +70,72 @@ def test_only_lane_zero_is_accepted():
     assert L.rcs_set_lane(1) < 0
     assert b"lane 0 only" in L.rcs_last_error()
     assert L.rcs_set_lane(64) < 0
+
+
+def test_bad_arguments_are_errors_before_any_device_work():
+    """Every entry point checks its arguments on the host before its first device call, so
+    on a machine without a GPU each bad call below fails with its own message, not with
+    'no HIP device' (and on a GPU it never reaches a kernel)."""
+    import rcs_amd
+    L = rcs_amd.lib()
+    C = ctypes
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def err(rc):
+        assert rc < 0
+        return L.rcs_last_error().decode()
+
+    z = np.zeros(64)
+    zi = np.zeros(16, dtype=np.int32)
+    zf = np.zeros(64, dtype=np.float32)
+    h = C.c_void_p()
+    # dense kernels: sizes, operations, leading dimensions, null buffers
+    assert "negative" in err(L.rcs_gemm(0, 0, 0, -1, 2, 2, dp(z), 1, dp(z), 2, dp(z)))
+    assert "op must be" in err(L.rcs_gemm(0, 2, 0, 2, 2, 2, dp(z), 2, dp(z), 2, dp(z)))
+    assert "leading dimension" in err(L.rcs_gemm(0, 0, 0, 4, 2, 2, dp(z), 3, dp(z), 2, dp(z)))
+    assert "leading dimension" in err(L.rcs_gemm(0, 1, 1, 4, 2, 3, dp(z), 2, dp(z), 2, dp(z)))
+    assert "A is null" in err(L.rcs_gemm(0, 0, 0, 2, 2, 2, None, 2, dp(z), 2, dp(z)))
+    assert "C is null" in err(L.rcs_gemm(0, 0, 0, 2, 2, 2, dp(z), 2, dp(z), 2, None))
+    assert "partition out of range" in err(L.rcs_rrqr(0, dp(z), 2, 2, 10, 10, 3, 0.0, 0, dp(z), ip(zi), dp(z)))
+    assert "negative" in err(L.rcs_rrqr(0, dp(z), -2, 2, 10, 10, 0, 0.0, 0, dp(z), ip(zi), dp(z)))
+    assert "rank is null" in err(L.rcs_rrqr(0, dp(z), 2, 2, 10, 10, 0, 0.0, 0, dp(z), None, dp(z)))
+    assert "negative" in err(L.rcs_lu_solve(0, dp(z), -1, dp(z), 1, dp(z)))
+    assert "X is null" in err(L.rcs_lu_solve(0, dp(z), 2, dp(z), 1, None))
+    # meshes: kind, frequency, sizes, null arrays, node numbers (1-based, as in the reference files)
+    nodes = np.eye(3)
+    tris = np.array([[1, 2, 3]], dtype=np.int32)
+    bad = np.array([[1, 2, 4]], dtype=np.int32)
+    assert "kernel kind" in err(L.rcs_mesh_create(dp(nodes), 3, ip(tris), 1, None, 7, 1e9, C.byref(h)))
+    assert "frequency" in err(L.rcs_mesh_create(dp(nodes), 3, ip(tris), 1, None, 0, 0.0, C.byref(h)))
+    assert "at least 3 nodes" in err(L.rcs_mesh_create(dp(nodes), 2, ip(tris), 1, None, 0, 1e9, C.byref(h)))
+    assert "nodes is null" in err(L.rcs_mesh_create(None, 3, ip(tris), 1, None, 0, 1e9, C.byref(h)))
+    assert "out is null" in err(L.rcs_mesh_create(dp(nodes), 3, ip(tris), 1, None, 0, 1e9, None))
+    assert "outside the node list" in err(L.rcs_mesh_create(dp(nodes), 3, ip(bad), 1, None, 0, 1e9, C.byref(h)))
+    # null handles and solver parameters
+    assert "mesh is null" in err(L.rcs_entries(None, 0, ip(zi), 1, ip(zi), 1, dp(z)))
+    assert "mesh is null" in err(L.rcs_aca(None, 0, 1e-3, ip(zi), 1, ip(zi), 1, ip(zi), ip(zi), ip(zi)))
+    assert "mesh is null" in err(L.rcs_rhs(None, dp(z), dp(z), dp(z)))
+    assert "mesh is null" in err(L.rcs_mesh_locations(None, dp(z)))
+    bb = np.array([-1.0, 1, -1, 1, -1, 1])
+    assert "mesh is null" in err(L.rcs_solver_create(None, dp(bb), 64, 6, 6, 6, 0, 0, C.byref(h)))
+    assert "variant" in err(L.rcs_solver_create(None, dp(bb), 64, 6, 6, 6, 0, 2, C.byref(h)))
+    assert "precision" in err(L.rcs_solver_create(None, dp(bb), 64, 6, 6, 6, 5, 0, C.byref(h)))
+    assert "leaf size" in err(L.rcs_solver_create(None, dp(bb), 0, 6, 6, 6, 0, 0, C.byref(h)))
+    flat = np.array([0.0, 0, 0, 0, 0, 0])
+    assert "bbox is empty" in err(L.rcs_solver_create(None, dp(flat), 64, 6, 6, 6, 0, 0, C.byref(h)))
+    swapped = np.array([1.0, -1, -1, 1, -1, 1])
+    assert "min <= max" in err(L.rcs_operator_create(None, dp(swapped), 64, 10, C.byref(h)))
+    assert "solver is null" in err(L.rcs_solver_factorize(None))
+    assert "solver is null" in err(L.rcs_solver_solve(None, dp(z), 1, dp(z), None))
+    assert "solver is null" in err(L.rcs_solver_stats(None, ip(zi), dp(z)))
+    assert "solver is null" in err(L.rcs_solver_ranks(None, 0, ip(zi), 16))
+    assert "solver is null" in err(L.rcs_solver_rank_log(None, fp(zf), 1))
+    assert "operator is null" in err(L.rcs_operator_apply(None, dp(z), 1, dp(z)))
+    assert "solver is null" in err(L.rcs_solver_refine(None, None, dp(z), 1, dp(z), 1e-6, 10, 10, None, None))
+    assert "model is null" in err(L.rcs_mlp_forward(None, fp(zf), 1, fp(zf)))
+    assert "model is null" in err(L.rcs_mlp_train(None, fp(zf), fp(zf), 1, 1, 0.01, None))
+    assert "out is null" in err(L.rcs_mlp_create(1, None))
+    assert "profile tag" in err(L.rcs_profile_get(99, dp(z), dp(z), None))
+    assert "burst" in err(L.rcs_set_aca_burst(0))
