@@ -188,6 +188,37 @@ def test_aca_stop_rule_cross_term_known_answer():
             assert O.aca_dense(B, thr * 0.999)[0] == 3
 
 
+@pytest.mark.parametrize("m,n,r", [(30, 40, 5), (40, 30, 5), (50, 50, 2), (20, 60, 8)])
+def test_aca_exact_rank_synthetic(rng, m, n, r):
+    """Exact-rank recovery on a random complex rank-r matrix (r >= 2): the first r pivots
+    form a nonsingular r x r intersection, and any such CUR reproduces a rank-r matrix
+    exactly. The reference's rule then takes one more pair made of rounding noise (its
+    |u||v| ~ 1e-16 |S| fails the test only after it is added, src/ACA.cpp:702,747), so the
+    returned rank is r + 1."""
+    for _ in range(3):
+        A = (rng.normal(size=(m, r)) + 1j * rng.normal(size=(m, r))) @ \
+            (rng.normal(size=(r, n)) + 1j * rng.normal(size=(r, n)))
+        k, rb, cb = O.aca_dense(A, 1e-8)
+        assert k == r + 1
+        C, D, R_ = A[:, cb[:r]], A[np.ix_(rb[:r], cb[:r])], A[rb[:r], :]
+        assert np.linalg.norm(A - C @ np.linalg.solve(D, R_)) < 1e-12 * np.linalg.norm(A)
+
+
+def test_aca_rank_one_runs_on_noise():
+    """Rank one is the degenerate case of the reference's rule: |S| never holds the first
+    pair, so after it only rounding-noise pairs are compared with each other and the loop
+    goes on until a noise residual vanishes or min(N1, N2) is reached. The first pivot is
+    still exact."""
+    rng = np.random.default_rng(11)
+    a = rng.normal(size=50) + 1j * rng.normal(size=50)
+    b = rng.normal(size=50) + 1j * rng.normal(size=50)
+    A = np.outer(a, b)
+    k, rb, cb = O.aca_dense(A, 1e-8)
+    assert k >= 2
+    i, j = rb[0], cb[0]
+    assert np.linalg.norm(A - np.outer(A[:, j], A[i, :]) / A[i, j]) < 1e-13 * np.linalg.norm(A)
+
+
 def test_aca_zero_rows_and_zero_matrix():
     """src/ACA.cpp:669-684: the first pivot row is the first row with an entry above 1e-16;
     an all-zero block has rank 0.
