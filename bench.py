@@ -35,7 +35,21 @@ CONFIGS = {
 }
 
 
-def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops):
+def reference_map_walk(level, radius, freq, samples=2000):
+    """The reference finds basis functions i and j of every entry by walking its std::map of
+    edges (src/EFIEMatrix.cpp:698-713); the restatement uses a table. Both costs per entry,
+    timed by the oracle on one thread on the benchmark's own mesh, show how much faster the
+    restatement's assembly is than the reference's would be."""
+    from oracle import pyoracle as O
+    from rcs_amd import mesh as M
+    nodes, tris = M.icosphere(level, radius)
+    m = O.Mesh(nodes, tris)
+    walk, entry = m.map_walk_cost(freq, samples, 1)
+    return {"N": m.n_rwg, "samples": samples, "lookup_us_per_entry": walk * 1e6,
+            "port_entry_us": entry * 1e6, "lookup_over_entry": walk / entry}
+
+
+def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops, level_bench=None):
     """Time the oracle (the FP64 CPU restatement) on a smaller sphere of the same family
     and scale its time to the benchmark size by the algorithmic flop ratio."""
     from oracle import pyoracle as O
@@ -52,7 +66,7 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops):
     inf = s.info()
     flops = inf["gemm_flops"] + inf["getrf_flops"] + inf["getrs_flops"] + inf["qr_flops"]
     est = wall * gpu_flops / max(flops, 1.0)
-    return {
+    out = {
         "value": 1.0 / est,
         "unit": "assemble+factorize+solve/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
@@ -64,6 +78,12 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops):
         "sample_wall_s": wall,
         "sample_t_assemble": inf["t_assemble"], "sample_t_factor": inf["t_factor"], "sample_t_solve": inf["t_solve"],
     }
+    if level_bench is not None:
+        try:
+            out["reference_map_walk"] = reference_map_walk(level_bench, radius, freq)
+        except Exception as e:
+            out["reference_map_walk"] = {"error": repr(e)}
+    return out
 
 
 def measured_traffic(config):
@@ -237,7 +257,7 @@ def main():
             log(f"accuracy / refined-forms section failed: {e!r}")
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_level, radius, freq, tol, flops, level)
         except Exception as e:
             out["cpu_baseline"] = {"error": repr(e)}
             log(f"CPU baseline failed: {e!r}")

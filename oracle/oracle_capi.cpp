@@ -145,6 +145,66 @@ int orc_aca(void* h, int kind, double freq, const int* rows, int nr, const int* 
   ORC_CATCH
 }
 
+// Cost of the reference's basis-function lookup, which the restatement replaces by a table.
+// EFIEMatrix keeps the edges in a std::map keyed by the sorted node pair, filled patch by
+// patch (E:127-203); getMatrixEntry finds basis i and j with std::next(map.begin(), i) and
+// std::next(map.begin(), j) and then looks both edges up four times (E:698-713). This times
+// that lookup alone, and the restatement's whole EFIE entry, on the same `samples` random
+// (i, j) pairs on one thread: seconds per entry in *walk_s and *entry_s.
+int orc_map_walk_cost(void* h, double freq, int samples, unsigned seed, double* walk_s, double* entry_s) {
+  ORC_TRY
+  const Mesh& mesh = ((OMesh*)h)->mesh;
+  struct EdgeProps {
+    double length;
+    std::vector<int> patches;
+  };
+  std::map<std::pair<int, int>, EdgeProps> edges;
+  for (size_t p = 0; p < mesh.patches.size(); ++p) {
+    const int v[3] = {mesh.patches[p].v1, mesh.patches[p].v2, mesh.patches[p].v3};
+    for (int e = 0; e < 3; ++e) {
+      const int a = v[e], b = v[(e + 1) % 3];
+      auto key = std::make_pair(std::min(a, b), std::max(a, b));
+      auto it = edges.find(key);
+      if (it == edges.end()) {
+        const V3 d = mesh.nodes[a] - mesh.nodes[b];
+        EdgeProps ep;
+        ep.length = std::sqrt(d.dot(d));
+        ep.patches.push_back((int)p);
+        edges[key] = ep;
+      } else {
+        it->second.patches.push_back((int)p);
+      }
+    }
+  }
+  for (auto it = edges.begin(); it != edges.end();) it = it->second.patches.size() == 1 ? edges.erase(it) : std::next(it);
+  const int N = (int)edges.size();
+  if (N != (int)mesh.rwg.size()) throw std::runtime_error("edge map and RWG table differ in size");
+  std::vector<int> I(samples), J(samples);
+  unsigned s = seed ? seed : 1u;
+  for (int k = 0; k < samples; ++k) {  // xorshift32
+    s ^= s << 13; s ^= s >> 17; s ^= s << 5; I[k] = (int)(s % (unsigned)N);
+    s ^= s << 13; s ^= s >> 17; s ^= s << 5; J[k] = (int)(s % (unsigned)N);
+  }
+  double sink = 0;
+  double t0 = now();
+  for (int k = 0; k < samples; ++k) {
+    auto ti = std::next(edges.begin(), I[k]);
+    auto sj = std::next(edges.begin(), J[k]);
+    const std::pair<int, int> te = ti->first, se = sj->first;
+    sink += edges[te].patches[0] + edges[te].patches[1] + edges[se].patches[0] + edges[se].patches[1];
+    sink += ti->second.length + sj->second.length + te.first + se.second;
+  }
+  *walk_s = (now() - t0) / std::max(samples, 1);
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, 0, freq));
+  cd acc = 0;
+  t0 = now();
+  for (int k = 0; k < samples; ++k) acc += K->entry(I[k], J[k]);
+  *entry_s = (now() - t0) / std::max(samples, 1);
+  if (sink == -1.0 && acc == cd(-1.0)) throw std::runtime_error("unreachable");  // keeps both loops
+  return 0;
+  ORC_CATCH
+}
+
 // The same ACA over all rows and columns of a column-major m x n complex matrix A.
 int orc_aca_dense(const double* A, int m, int n, double tol, int* rank, int* row_bases, int* col_bases) {
   ORC_TRY

@@ -38,6 +38,7 @@ def lib():
         L.orc_rcs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int, dp]
         L.orc_aca.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, C.c_double, ip, ip, ip]
         L.orc_aca_dense.argtypes = [dp, C.c_int, C.c_int, C.c_double, ip, ip, ip]
+        L.orc_map_walk_cost.argtypes = [vp, C.c_double, C.c_int, C.c_uint, dp, dp]
         L.orc_rrqr.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                dp, dp, ip, ip, dp]
         L.orc_lu_solve.argtypes = [dp, C.c_int, dp, C.c_int, dp]
@@ -142,6 +143,14 @@ class Mesh:
                              _ip(rk), _ip(rb), _ip(cb)))
         r = int(rk[0])
         return r, rb[:r].copy(), cb[:r].copy()
+
+    def map_walk_cost(self, freq, samples=2000, seed=1):
+        """Seconds per entry of the reference's std::map basis lookup and of the restatement's
+        whole EFIE entry, timed on the same random (i, j) pairs on one thread."""
+        w = np.zeros(1)
+        e = np.zeros(1)
+        _check(lib().orc_map_walk_cost(self.h, freq, samples, seed, _dp(w), _dp(e)))
+        return float(w[0]), float(e[0])
 
     def dense_solve(self, kind, freq, rhs):
         rhs = np.ascontiguousarray(rhs, dtype=np.complex128)

@@ -387,3 +387,12 @@ def test_rank_mlp_oracle_gradient_finite_differences():
     # BF16 rounding: 1 + 2^-8 is a tie (rounds to even 1.0); 1 + 3*2^-9 rounds up to 1 + 2^-7
     assert Q.bf16_round(np.array([1.0 + 2 ** -8], np.float32))[0] == 1.0
     assert Q.bf16_round(np.array([1.0 + 3 * 2 ** -9], np.float32))[0] == np.float32(1.0 + 2 ** -7)
+
+
+def test_reference_edge_map_lookup_cost_runs(fixture78):
+    """The oracle's model of the reference's edge std::map (E:127-209) keeps exactly the RWG
+    edges of the fixture, and both per-entry costs come back positive."""
+    nodes, tris = fixture78
+    m = O.Mesh(nodes, tris)
+    walk, entry = m.map_walk_cost(1.0e9, 200, 3)
+    assert walk > 0 and entry > 0
