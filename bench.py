@@ -86,6 +86,29 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops, level_bench=None):
     return out
 
 
+def solve_roofline(factor_bytes, n, prec, t_1rhs, t_181rhs):
+    """HBM figure of the solve: the algorithmic bytes are every stored factor block read once
+    plus the right-hand sides read and the solutions written (2 N nrhs complex values), over
+    the host wall time of rcs_solver_solve (which includes the RHS upload and the solution
+    download, so the rate is a lower bound)."""
+    vb = 16 if prec == 0 else 8
+    out = {"bound": "hbm", "peak": PEAK_HBM / 1e9, "unit": "GB/s", "factor_bytes": factor_bytes,
+           "timer": "host wall time of rcs_solver_solve"}
+    for tag, nrhs, t in (("1rhs", 1, t_1rhs), ("181rhs", 181, t_181rhs)):
+        b = factor_bytes + 2.0 * n * nrhs * vb
+        a = b / t if t and t > 0 else 0.0
+        out[f"bytes_{tag}"] = b
+        out[f"achieved_{tag}"] = a / 1e9
+        out[f"frac_{tag}"] = a / PEAK_HBM
+    # with 181 right-hand sides every factor entry does 8 flop (one complex FMA) per column
+    fl = 8.0 * (factor_bytes / vb) * 181
+    peak = PEAK_FP64 if prec == 0 else PEAK_FP32
+    out["flops_181rhs"] = fl
+    out["achieved_tflops_181rhs"] = fl / t_181rhs / 1e12 if t_181rhs and t_181rhs > 0 else 0.0
+    out["frac_flops_181rhs"] = out["achieved_tflops_181rhs"] * 1e12 / peak
+    return out
+
+
 def measured_traffic(config):
     """HBM bytes per launch of the Schur update kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size); None when no
@@ -238,6 +261,10 @@ def main():
                                 "work": v["flops"] / args.steps} for k, v in prof.items()},
         "rcs_theta0_dbsm": float(10 * np.log10(rcs[0])),
     }
+    try:
+        out["solve_roofline"] = solve_roofline(st["factor_bytes"], g.N, prec, st["t_solve"], t181)
+    except Exception as e:
+        out["solve_roofline"] = {"error": repr(e)}
     own_log = s.rank_log()
     del s
     net = None

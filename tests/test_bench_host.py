@@ -1,0 +1,34 @@
+"""Host-side arithmetic of bench.py (no GPU): the solve's roofline figures and the refusal of
+more than one GPU."""
+import os
+import subprocess
+import sys
+
+import bench
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_solve_roofline_by_hand():
+    # 1 GB of factors, N = 1000, complex128: one RHS moves 1e9 + 2*1000*16 bytes in 0.5 s
+    r = bench.solve_roofline(1.0e9, 1000, 0, 0.5, 2.0)
+    assert r["bytes_1rhs"] == 1.0e9 + 32000
+    assert abs(r["achieved_1rhs"] - (1.0e9 + 32000) / 0.5 / 1e9) < 1e-12
+    assert abs(r["frac_1rhs"] - (1.0e9 + 32000) / 0.5 / 8.0e12) < 1e-15
+    assert r["bytes_181rhs"] == 1.0e9 + 2 * 1000 * 181 * 16
+    # 1e9 / 16 complex128 entries, 8 flop each per column, 181 columns, 2 s
+    assert abs(r["achieved_tflops_181rhs"] - 8 * 1.0e9 / 16 * 181 / 2.0 / 1e12) < 1e-12
+    assert abs(r["frac_flops_181rhs"] - r["achieved_tflops_181rhs"] * 1e12 / 78.6e12) < 1e-15
+    # complex64 halves the vector bytes and uses the FP32 peak
+    r = bench.solve_roofline(1.0e9, 1000, 1, 0.5, 2.0)
+    assert r["bytes_1rhs"] == 1.0e9 + 16000
+    assert abs(r["frac_flops_181rhs"] - r["achieved_tflops_181rhs"] * 1e12 / 157.3e12) < 1e-15
+    # a zero time reports zero, not a division error
+    assert bench.solve_roofline(1.0e9, 1000, 0, 0.0, 0.0)["achieved_1rhs"] == 0.0
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "exactly one GPU" in r.stderr
