@@ -29,10 +29,16 @@ const char* rcs_last_error(void);
 /* EFIEMatrix(inputsToKernelClass) (src/EFIEMatrix.cpp:5-20, mesh ingest and RWG
  * numbering E:31-209) or ElectrostaticsMatrix(inputsToKernelClass)
  * (src/ElectrostaticsMatrix.cpp:5-9). nodes: n_nodes x 3 (the node file lines);
- * tris: n_tris x 3, 1-based node indices (the patch file lines); conductor may be NULL.
+ * tris: n_tris x 3, 1-based node indices (the patch file lines); conductor (the optional
+ * fourth column, E:78-79) may be NULL, which gives every patch conductor 0 as there.
  * freq replaces the compile-time constant of include/includes.hpp:58. */
 int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
                     int kernel_kind, double freq, rcs_mesh_t* out);
+/* The same mesh with host tables only (no device, no HIP call): enough for the numbering,
+ * patches, right-hand side and RCS entry points below; rcs_entries, rcs_aca,
+ * rcs_solver_create and rcs_operator_create reject it. */
+int rcs_mesh_create_host(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
+                         int kernel_kind, double freq, rcs_mesh_t* out);
 void rcs_mesh_destroy(rcs_mesh_t m);
 /* EFIEMatrix::N (E:97-99) / ElectrostaticsMatrix::N. */
 int rcs_mesh_size(rcs_mesh_t m);
@@ -42,6 +48,11 @@ int rcs_mesh_locations(rcs_mesh_t m, double* xyz);
 /* RWG table in basis order: n1 n2 T+ T- free+ free- per basis (edge2PatchMap,
  * E:129-209 and getIndexOfThirdNode E:353-373), plus the edge lengths. */
 int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length);
+/* Patch count and patchList (include/Patch.hpp; centroid, area and largestEdge from
+ * E:100-124, the reference's largestEdge looking at two of the three edges): out5 holds
+ * centroid x y z, area, largestEdge per patch; conductor (may be NULL) the conductor index. */
+int rcs_mesh_npatch(rcs_mesh_t m);
+int rcs_mesh_patches(rcs_mesh_t m, double* out5, int* conductor);
 
 /* kernel::getMatrix(rows, cols) (src/kernel.cpp:49-61) over getMatrixEntry
  * (E:698-783 / src/ElectrostaticsMatrix.cpp:293-340), computed on the GPU in the
@@ -54,6 +65,11 @@ int rcs_rhs(rcs_mesh_t m, const double* E, const double* kvec, double* out);
 
 /* getJAtCentroids (E:1100-1130) + getRCS_OnePointQuadrature (E:916-930) at n angles. */
 int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double* phi, int n, double* out);
+/* EFIEMatrix::getJAtCentroids (E:1100-1130): J is n_patch x 3 complex, column-major (the
+ * reference's MatrixXcd JAtCentroids). */
+int rcs_j_at_centroids(rcs_mesh_t m, const double* coef, double* J);
+/* EFIEMatrix::getRCS_OnePointQuadrature(theta, phi, JAtCentroids) (E:916-930) at n angles, m^2. */
+int rcs_rcs_from_j(rcs_mesh_t m, const double* J, const double* theta, const double* phi, int n, double* out);
 
 /* LowRank<K>(K, tol, rows, cols).ACA_only_nodes2 (include/ACA.hpp:16,26; src/ACA.cpp:639-859)
  * on the GPU. row_bases/col_bases must hold min(nr, nc) entries. */

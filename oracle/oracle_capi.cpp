@@ -121,6 +121,21 @@ int orc_rcs(void* h, double freq, const double* coef, const double* theta, const
   ORC_CATCH
 }
 
+// getJAtCentroids (E:1100-1130): J is n_patch x 3 complex, column-major.
+int orc_j_at_centroids(void* h, double freq, const double* coef, double* J) {
+  ORC_TRY
+  EFIE K(&((OMesh*)h)->mesh, freq);
+  std::vector<cd> c(K.size());
+  std::memcpy(c.data(), coef, sizeof(cd) * c.size());
+  std::vector<C3> j;
+  K.jAtCentroids(c, j);
+  const size_t F = j.size();
+  cd* o = reinterpret_cast<cd*>(J);
+  for (size_t t = 0; t < F; ++t) { o[t] = j[t].x; o[t + F] = j[t].y; o[t + 2 * F] = j[t].z; }
+  return 0;
+  ORC_CATCH
+}
+
 // Closed-form panel integrals of 1/R and (r'-p)/R over triangle (a, b, c) seen from tc, p
 // the projection of tc on the triangle's plane
 // (E:478-668), for the quadrature cross-checks.

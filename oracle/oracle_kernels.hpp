@@ -65,7 +65,7 @@ struct Mesh {
       p.v1 = tris[3 * t];
       p.v2 = tris[3 * t + 1];
       p.v3 = tris[3 * t + 2];
-      p.conductor = conductor ? conductor[t] : 1;
+      p.conductor = conductor ? conductor[t] : 0;  // E:78-79: a patch line without the column reads 0
       conductorIndices.insert(p.conductor);
       const V3 &a = nodes[p.v1], &b = nodes[p.v2], &c = nodes[p.v3];
       p.area = area(a, b, c);

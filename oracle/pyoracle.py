@@ -36,6 +36,7 @@ def lib():
         L.orc_entries.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, dp]
         L.orc_rhs.argtypes = [vp, C.c_double, dp, dp, dp]
         L.orc_rcs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int, dp]
+        L.orc_j_at_centroids.argtypes = [vp, C.c_double, dp, dp]
         L.orc_aca.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, C.c_double, ip, ip, ip]
         L.orc_aca_dense.argtypes = [dp, C.c_int, C.c_int, C.c_double, ip, ip, ip]
         L.orc_map_walk_cost.argtypes = [vp, C.c_double, C.c_int, C.c_uint, dp, dp]
@@ -131,6 +132,13 @@ class Mesh:
         _check(lib().orc_rcs(self.h, freq, coef.ctypes.data_as(C.POINTER(C.c_double)), _dp(theta), _dp(phi),
                              theta.size, _dp(out)))
         return out
+
+    def j_at_centroids(self, freq, coef):
+        coef = np.ascontiguousarray(coef, dtype=np.complex128)
+        J = np.zeros((self.n_patch, 3), dtype=np.complex128, order="F")
+        _check(lib().orc_j_at_centroids(self.h, freq, coef.ctypes.data_as(C.POINTER(C.c_double)),
+                                        J.ctypes.data_as(C.POINTER(C.c_double))))
+        return J
 
     def aca(self, kind, freq, rows, cols, tol):
         rows = np.ascontiguousarray(rows, dtype=np.int32)

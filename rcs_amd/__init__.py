@@ -34,6 +34,11 @@ def lib():
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
         L.rcs_last_error.restype = C.c_char_p
         L.rcs_mesh_create.argtypes = [dp, C.c_int, ip, C.c_int, ip, C.c_int, C.c_double, C.POINTER(vp)]
+        L.rcs_mesh_create_host.argtypes = [dp, C.c_int, ip, C.c_int, ip, C.c_int, C.c_double, C.POINTER(vp)]
+        L.rcs_mesh_npatch.argtypes = [vp]
+        L.rcs_mesh_patches.argtypes = [vp, dp, ip]
+        L.rcs_j_at_centroids.argtypes = [vp, dp, dp]
+        L.rcs_rcs_from_j.argtypes = [vp, dp, dp, dp, C.c_int, dp]
         L.rcs_mesh_destroy.argtypes = [vp]
         L.rcs_mesh_size.argtypes = [vp]
         L.rcs_mesh_locations.argtypes = [vp, dp]
@@ -77,8 +82,9 @@ def lib():
 
 
 EXPORTED = [
-    "rcs_init", "rcs_last_error", "rcs_mesh_create", "rcs_mesh_destroy", "rcs_mesh_size", "rcs_mesh_locations",
-    "rcs_mesh_rwg", "rcs_entries", "rcs_rhs", "rcs_rcs", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
+    "rcs_init", "rcs_last_error", "rcs_mesh_create", "rcs_mesh_create_host", "rcs_mesh_destroy", "rcs_mesh_size",
+    "rcs_mesh_locations", "rcs_mesh_rwg", "rcs_mesh_npatch", "rcs_mesh_patches", "rcs_entries", "rcs_rhs", "rcs_rcs",
+    "rcs_j_at_centroids", "rcs_rcs_from_j", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
     "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
     "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
     "rcs_operator_create", "rcs_operator_destroy", "rcs_operator_apply", "rcs_solver_refine",
@@ -122,16 +128,20 @@ def _check(rc):
 
 
 class Mesh:
-    """EFIEMatrix / ElectrostaticsMatrix over the GPU geometry tables."""
+    """EFIEMatrix / ElectrostaticsMatrix over the GPU geometry tables. With host_only=True
+    the mesh keeps host tables only (no device is touched): numbering, patches, right-hand
+    sides and RCS work, entries, ACA and solvers do not."""
 
-    def __init__(self, nodes, tris, kind=EFIE, freq=1.0e9, conductor=None):
+    def __init__(self, nodes, tris, kind=EFIE, freq=1.0e9, conductor=None, host_only=False):
         nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         tris = np.ascontiguousarray(tris, dtype=np.int32)
         cond = None if conductor is None else np.ascontiguousarray(conductor, dtype=np.int32)
         h = C.c_void_p()
-        _check(lib().rcs_mesh_create(_dp(nodes), nodes.shape[0], _ip(tris), tris.shape[0],
-                                     None if cond is None else _ip(cond), kind, freq, C.byref(h)))
+        create = lib().rcs_mesh_create_host if host_only else lib().rcs_mesh_create
+        _check(create(_dp(nodes), nodes.shape[0], _ip(tris), tris.shape[0], None if cond is None else _ip(cond),
+                      kind, freq, C.byref(h)))
         self.h = h
+        self.n_patch = lib().rcs_mesh_npatch(h)
         self.kind = kind
         self.freq = freq
         self.N = lib().rcs_mesh_size(h)
@@ -176,6 +186,29 @@ class Mesh:
                              _dp(phi), theta.size, _dp(out)))
         return out
 
+    def patches(self):
+        """(centroid xyz, area, largestEdge) per patch and the conductor indices."""
+        out = np.zeros((self.n_patch, 5))
+        cond = np.zeros(self.n_patch, dtype=np.int32)
+        _check(lib().rcs_mesh_patches(self.h, _dp(out), _ip(cond)))
+        return out, cond
+
+    def j_at_centroids(self, coef):
+        """getJAtCentroids: n_patch x 3 complex surface current."""
+        coef = np.ascontiguousarray(coef, dtype=np.complex128)
+        J = np.zeros((self.n_patch, 3), dtype=np.complex128, order="F")
+        _check(lib().rcs_j_at_centroids(self.h, _cplx_ptr(coef), _cplx_ptr_f(J)))
+        return J
+
+    def rcs_from_j(self, J, theta, phi):
+        """getRCS_OnePointQuadrature at each (theta, phi)."""
+        J = np.asfortranarray(J, dtype=np.complex128)
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.zeros(theta.size)
+        _check(lib().rcs_rcs_from_j(self.h, _cplx_ptr_f(J), _dp(theta), _dp(phi), theta.size, _dp(out)))
+        return out
+
     def aca(self, rows, cols, tol, precision=C128):
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
@@ -191,6 +224,12 @@ class Mesh:
 
 def _cplx_ptr(a):
     return a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _cplx_ptr_f(a):
+    """Pointer to a Fortran-ordered complex array's memory (column-major, interleaved)."""
+    assert a.flags.f_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def rrqr(A, tol_pow1, tol_pow2, partition, input_pivot, fillin_rank, precision=C128):

@@ -18,6 +18,7 @@ using namespace rcs;
 
 struct rcs_mesh_s {
   std::shared_ptr<DeviceGeometry> g;
+  bool on_device = false;  // false for rcs_mesh_create_host: host tables only
 };
 struct rcs_solver_s {
   std::unique_ptr<SolverBase> s;
@@ -76,6 +77,80 @@ void check_bbox(const double* bb) {
     if (!(bb[2 * d + 1] >= bb[2 * d])) throw std::runtime_error("bbox must list min <= max for x, y and z");
   if (!(std::max({bb[1] - bb[0], bb[3] - bb[2], bb[5] - bb[4]}) > 0)) throw std::runtime_error("bbox is empty");
 }
+void need_device(rcs_mesh_t m) {
+  need(m, "mesh");
+  if (!m->on_device) throw std::runtime_error("mesh was created by rcs_mesh_create_host and has no device tables");
+}
+
+// Surface current at the patch centroids, getJAtCentroids (E:1100-1130): every RWG basis adds
+// coef * rho * L / (2 A) to its T+ patch and the negative of it, with its own rho, to its T-
+// patch (rho = centroid - free vertex). J is F x 3 complex, column-major (interleaved re, im).
+void j_at_centroids(const HostGeometry& h, const double* coef, double* J) {
+  const size_t F = h.v1.size();
+  std::fill(J, J + 6 * F, 0.0);
+  for (size_t i = 0; i < h.n1.size(); ++i) {
+    const double cr = coef[2 * i], ci = coef[2 * i + 1];
+    const int P = h.tp[i], M = h.tm[i];
+    const double sp = h.len[i] / 2 / h.area[P], sm = -h.len[i] / 2 / h.area[M];
+    const double rp[3] = {h.cx[P] - h.nx[h.fp[i]], h.cy[P] - h.ny[h.fp[i]], h.cz[P] - h.nz[h.fp[i]]};
+    const double rm[3] = {h.cx[M] - h.nx[h.fm[i]], h.cy[M] - h.ny[h.fm[i]], h.cz[M] - h.nz[h.fm[i]]};
+    for (int d = 0; d < 3; ++d) {
+      double* jp = J + 2 * (P + d * F);
+      double* jm = J + 2 * (M + d * F);
+      jp[0] += cr * rp[d] * sp;
+      jp[1] += ci * rp[d] * sp;
+      jm[0] += cr * rm[d] * sm;
+      jm[1] += ci * rm[d] * sm;
+    }
+  }
+}
+
+// Monostatic RCS from the centroid currents, getRCS_OnePointQuadrature (E:916-930), m^2:
+// k^2 / (4 pi) * eta^2 * (|sum J_theta e^{i k.c} A|^2 + |sum J_phi e^{i k.c} A|^2), eta = 377.
+void rcs_from_j(const HostGeometry& h, const double* J, const double* theta, const double* phi, int n, double* out) {
+  const size_t F = h.v1.size();
+  const double PI = 4.0 * std::atan(1.0);
+  const double kap = 2 * PI / (3.0e8 / h.freq);
+  const int neta = 377;  // E:917
+  for (int a = 0; a < n; ++a) {
+    const double th = theta[a], ph = phi[a];
+    const double kv[3] = {kap * std::sin(th) * std::cos(ph), kap * std::sin(th) * std::sin(ph), kap * std::cos(th)};
+    double str = 0, sti = 0, spr = 0, spi = 0;
+    for (size_t t = 0; t < F; ++t) {
+      const double ph_ = kv[0] * h.cx[t] + kv[1] * h.cy[t] + kv[2] * h.cz[t];
+      const double c = std::cos(ph_), s = std::sin(ph_), A = h.area[t];
+      const double* jx = J + 2 * t;
+      const double* jy = J + 2 * (t + F);
+      const double* jz = J + 2 * (t + 2 * F);
+      const double ar = jx[0] * std::cos(th) * std::cos(ph) + jy[0] * std::cos(th) * std::sin(ph) - jz[0] * std::sin(th);
+      const double ai = jx[1] * std::cos(th) * std::cos(ph) + jy[1] * std::cos(th) * std::sin(ph) - jz[1] * std::sin(th);
+      str += (ar * c - ai * s) * A;
+      sti += (ar * s + ai * c) * A;
+      const double br = -jx[0] * std::sin(ph) + jy[0] * std::cos(ph), bi = -jx[1] * std::sin(ph) + jy[1] * std::cos(ph);
+      spr += (br * c - bi * s) * A;
+      spi += (br * s + bi * c) * A;
+    }
+    const double at = std::sqrt(str * str + sti * sti), ap = std::sqrt(spr * spr + spi * spi);
+    out[a] = kap * kap / (4.0 * PI) * neta * at * neta * at + kap * kap / (4.0 * PI) * neta * ap * neta * ap;
+  }
+}
+
+std::unique_ptr<rcs_mesh_s> build_mesh(const double* nodes, int n_nodes, const int* tris, int n_tris,
+                                       const int* conductor, int kernel_kind, double freq, rcs_mesh_t* out) {
+  if (kernel_kind != KIND_EFIE && kernel_kind != KIND_LAPLACE) throw std::runtime_error("unknown kernel kind");
+  if (!(freq > 0)) throw std::runtime_error("frequency must be positive");
+  if (n_nodes < 3 || n_tris < 1) throw std::runtime_error("a mesh needs at least 3 nodes and 1 triangle");
+  need(nodes, "nodes");
+  need(tris, "tris");
+  need(out, "out");
+  std::unique_ptr<rcs_mesh_s> m(new rcs_mesh_s);
+  m->g = std::make_shared<DeviceGeometry>();
+  m->g->h.kind = kernel_kind;
+  m->g->h.freq = freq;
+  m->g->h.build(nodes, n_nodes, tris, n_tris, conductor);
+  return m;
+}
+
 template <typename T>
 DMat<T> dalloc(Ctx& cx, int r, int c) {
   DMat<T> m;
@@ -241,19 +316,16 @@ int rcs_set_lane(int lane) {
 
 int rcs_mesh_create(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
                     int kernel_kind, double freq, rcs_mesh_t* out) {
-  TRY if (kernel_kind != KIND_EFIE && kernel_kind != KIND_LAPLACE) throw std::runtime_error("unknown kernel kind");
-  if (!(freq > 0)) throw std::runtime_error("frequency must be positive");
-  if (n_nodes < 3 || n_tris < 1) throw std::runtime_error("a mesh needs at least 3 nodes and 1 triangle");
-  need(nodes, "nodes");
-  need(tris, "tris");
-  need(out, "out");
-  std::unique_ptr<rcs_mesh_s> m(new rcs_mesh_s);
-  m->g = std::make_shared<DeviceGeometry>();
-  m->g->h.kind = kernel_kind;
-  m->g->h.freq = freq;
-  m->g->h.build(nodes, n_nodes, tris, n_tris, conductor);
+  TRY std::unique_ptr<rcs_mesh_s> m = build_mesh(nodes, n_nodes, tris, n_tris, conductor, kernel_kind, freq, out);
   m->g->upload(ctx());
+  m->on_device = true;
   *out = m.release();
+  return 0;
+  CATCH
+}
+int rcs_mesh_create_host(const double* nodes, int n_nodes, const int* tris, int n_tris, const int* conductor,
+                         int kernel_kind, double freq, rcs_mesh_t* out) {
+  TRY *out = build_mesh(nodes, n_nodes, tris, n_tris, conductor, kernel_kind, freq, out).release();
   return 0;
   CATCH
 }
@@ -283,7 +355,7 @@ int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length) {
 }
 
 int rcs_entries(rcs_mesh_t m, int precision, const int* rows, int nr, const int* cols, int nc, double* out) {
-  TRY need(m, "mesh");
+  TRY need_device(m);
   const int N = m->g->h.n_unknowns();
   check_indices(rows, nr, N, "row");
   check_indices(cols, nc, N, "column");
@@ -325,50 +397,48 @@ int rcs_rcs(rcs_mesh_t m, const double* coef, const double* theta, const double*
   if (n > 0) { need(theta, "theta"); need(phi, "phi"); need(out, "out"); }
   const HostGeometry& h = m->g->h;
   if (h.kind != KIND_EFIE) throw std::runtime_error("rcs_rcs needs an EFIE mesh");
-  const size_t F = h.v1.size();
-  std::vector<double> J(6 * F, 0.0);  // (Jx, Jy, Jz) complex per patch
-  for (size_t i = 0; i < h.n1.size(); ++i) {
-    double cr = coef[2 * i], ci = coef[2 * i + 1];
-    int P = h.tp[i], M = h.tm[i];
-    double sp = h.len[i] / 2 / h.area[P], sm = -h.len[i] / 2 / h.area[M];
-    double rp[3] = {h.cx[P] - h.nx[h.fp[i]], h.cy[P] - h.ny[h.fp[i]], h.cz[P] - h.nz[h.fp[i]]};
-    double rm[3] = {h.cx[M] - h.nx[h.fm[i]], h.cy[M] - h.ny[h.fm[i]], h.cz[M] - h.nz[h.fm[i]]};
-    for (int d = 0; d < 3; ++d) {
-      J[6 * P + 2 * d] += cr * rp[d] * sp;
-      J[6 * P + 2 * d + 1] += ci * rp[d] * sp;
-      J[6 * M + 2 * d] += cr * rm[d] * sm;
-      J[6 * M + 2 * d + 1] += ci * rm[d] * sm;
-    }
-  }
-  const double PI = 4.0 * std::atan(1.0);
-  const double kap = 2 * PI / (3.0e8 / h.freq);
-  const int neta = 377;  // E:917
-  for (int a = 0; a < n; ++a) {
-    double th = theta[a], ph = phi[a];
-    double kv[3] = {kap * std::sin(th) * std::cos(ph), kap * std::sin(th) * std::sin(ph), kap * std::cos(th)};
-    double str = 0, sti = 0, spr = 0, spi = 0;
-    for (size_t t = 0; t < F; ++t) {
-      double ph_ = kv[0] * h.cx[t] + kv[1] * h.cy[t] + kv[2] * h.cz[t];
-      double c = std::cos(ph_), s = std::sin(ph_), A = h.area[t];
-      const double* j = &J[6 * t];
-      double ar = j[0] * std::cos(th) * std::cos(ph) + j[2] * std::cos(th) * std::sin(ph) - j[4] * std::sin(th);
-      double ai = j[1] * std::cos(th) * std::cos(ph) + j[3] * std::cos(th) * std::sin(ph) - j[5] * std::sin(th);
-      str += (ar * c - ai * s) * A;
-      sti += (ar * s + ai * c) * A;
-      double br = -j[0] * std::sin(ph) + j[2] * std::cos(ph), bi = -j[1] * std::sin(ph) + j[3] * std::cos(ph);
-      spr += (br * c - bi * s) * A;
-      spi += (br * s + bi * c) * A;
-    }
-    double at = std::sqrt(str * str + sti * sti), ap = std::sqrt(spr * spr + spi * spi);
-    out[a] = kap * kap / (4.0 * PI) * neta * at * neta * at + kap * kap / (4.0 * PI) * neta * ap * neta * ap;
+  std::vector<double> J(6 * h.v1.size());
+  j_at_centroids(h, coef, J.data());
+  rcs_from_j(h, J.data(), theta, phi, n, out);
+  return 0;
+  CATCH
+}
+int rcs_j_at_centroids(rcs_mesh_t m, const double* coef, double* J) {
+  TRY need(m, "mesh");
+  need(coef, "coef");
+  need(J, "J");
+  if (m->g->h.kind != KIND_EFIE) throw std::runtime_error("rcs_j_at_centroids needs an EFIE mesh");
+  j_at_centroids(m->g->h, coef, J);
+  return 0;
+  CATCH
+}
+int rcs_rcs_from_j(rcs_mesh_t m, const double* J, const double* theta, const double* phi, int n, double* out) {
+  TRY need(m, "mesh");
+  need(J, "J");
+  if (n < 0) throw std::runtime_error("n must not be negative");
+  if (n > 0) { need(theta, "theta"); need(phi, "phi"); need(out, "out"); }
+  if (m->g->h.kind != KIND_EFIE) throw std::runtime_error("rcs_rcs_from_j needs an EFIE mesh");
+  rcs_from_j(m->g->h, J, theta, phi, n, out);
+  return 0;
+  CATCH
+}
+int rcs_mesh_patches(rcs_mesh_t m, double* out5, int* conductor) {
+  TRY need(m, "mesh");
+  need(out5, "out5");
+  const HostGeometry& h = m->g->h;
+  for (size_t t = 0; t < h.v1.size(); ++t) {
+    double* o = out5 + 5 * t;
+    o[0] = h.cx[t]; o[1] = h.cy[t]; o[2] = h.cz[t]; o[3] = h.area[t]; o[4] = h.le[t];
+    if (conductor) conductor[t] = h.cond[t];
   }
   return 0;
   CATCH
 }
+int rcs_mesh_npatch(rcs_mesh_t m) { return m ? (int)m->g->h.v1.size() : -1; }
 
 int rcs_aca(rcs_mesh_t m, int precision, double tol, const int* rows, int nr, const int* cols, int nc, int* rank,
             int* row_bases, int* col_bases) {
-  TRY need(m, "mesh");
+  TRY need_device(m);
   need(rank, "rank");
   if (nr <= 0 || nc <= 0) { *rank = 0; return 0; }
   const int N = m->g->h.n_unknowns();
@@ -437,7 +507,7 @@ int rcs_solver_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_p
   if (variant != 0 && variant != 1) throw std::runtime_error("variant must be 0 or 1");
   if (precision != PREC_C128 && precision != PREC_C64) throw std::runtime_error("unknown precision");
   check_bbox(bbox);
-  need(m, "mesh");
+  need_device(m);
   need(out, "out");
   SolverParams p;
   for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];
@@ -557,7 +627,7 @@ int rcs_solver_solve(rcs_solver_t s, const double* rhs, int nrhs, double* x, dou
 int rcs_operator_create(rcs_mesh_t m, const double* bbox, int leaf_size, int tol_pow, rcs_operator_t* out) {
   TRY if (leaf_size <= 0) throw std::runtime_error("leaf size must be positive");
   check_bbox(bbox);
-  need(m, "mesh");
+  need_device(m);
   need(out, "out");
   SolverParams p;
   for (int i = 0; i < 6; ++i) p.bbox[i] = bbox[i];

@@ -40,7 +40,7 @@ struct HostGeometry {
       if (a < 1 || b < 1 || c < 1 || a > n_nodes || b > n_nodes || c > n_nodes)
         throw std::runtime_error("triangle references a node outside the node list");
       v1[t] = a; v2[t] = b; v3[t] = c;
-      cond[t] = conductor ? conductor[t] : 1;
+      cond[t] = conductor ? conductor[t] : 0;  // E:78-79: a patch line without the column reads 0
       double ax = nx[a], ay = ny[a], az = nz[a], bx = nx[b], by = ny[b], bz = nz[b], qx = nx[c], qy = ny[c], qz = nz[c];
       cx[t] = (ax + bx + qx) / 3.0; cy[t] = (ay + by + qy) / 3.0; cz[t] = (az + bz + qz) / 3.0;
       double e1x = bx - ax, e1y = by - ay, e1z = bz - az, e2x = qx - ax, e2y = qy - ay, e2z = qz - az;

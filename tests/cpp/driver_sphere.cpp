@@ -1,7 +1,7 @@
 // The reference driver's flow (test/sphere.cpp:80-262) written against the drop-in C++
 // interface include/rcs_amd.hpp: mesh files -> EFIEMatrix -> RWG midpoints and bounding
-// box -> inputsToSolverClass -> AIFMM -> factorize -> getRhs(Einc = x) -> solve ->
-// getPhi -> RCS. Also exercises LowRank::ACA_only_nodes2, RRQR and kernel::getMatrix.
+// box -> inputsToSolverClass -> AIFMM -> factorize -> getRhs(Einc = x) -> solve and
+// backSubstitute1/2/3 -> getPhi -> RCS. Also exercises LowRank::ACA_only_nodes2, RRQR and kernel::getMatrix.
 //
 // usage: driver_sphere nodeList patchList freq tol_pow leaf_size out_prefix
 // writes <out_prefix>_phi.txt (N lines "re im", input order), <out_prefix>_box.txt (box
@@ -61,7 +61,11 @@ int main(int argc, char** argv) {
 
     rcs::AIFMM<rcs::EFIEMatrix>* aifmm = new rcs::AIFMM<rcs::EFIEMatrix>(inputsToSolver, mykernel);
     aifmm->factorize();
-    rcs::Vec xbox = aifmm->solve(rhs);
+    rcs::Vec xbox = aifmm->solve(rhs);  // box order (src/AIFMM.cpp:102-110)
+    // the reference driver's own solve sequence (test/sphere.cpp:253-264)
+    aifmm->backSubstitute1(rhs);
+    aifmm->backSubstitute2();
+    aifmm->backSubstitute3();
     rcs::Vec phi;
     aifmm->getPhi(phi);
     write_vec(out + "_phi.txt", phi);

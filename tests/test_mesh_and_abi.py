@@ -139,3 +139,16 @@ def test_bad_arguments_are_errors_before_any_device_work():
     assert "out is null" in err(L.rcs_mlp_create(1, None))
     assert "profile tag" in err(L.rcs_profile_get(99, dp(z), dp(z), None))
     assert "burst" in err(L.rcs_set_aca_burst(0))
+
+
+def test_cpp_header_offers_the_reference_surface():
+    """tests/cpp/reference_surface.cpp uses every member the reference's driver and tree use
+    on its classes (EFIEMatrix's nodeList / edge2PatchMap / conductorIndices, getCoefficients,
+    getJAtCentroids, getRCS_OnePointQuadrature, AIFMM's backSubstitute1/2/3 and getPhi,
+    ElectrostaticsMatrix's getRhs / getchargeDensityUsingLU / getCapacitance, RRQR's
+    errorInQR ...) with the reference's names; it must compile against include/rcs_amd.hpp."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "reference_surface.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        src], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
