@@ -166,6 +166,10 @@ int rcs_solver_aca_stats(rcs_solver_t s, long long* steps, long long* syncs, dou
  * vals[0..10] = t_assemble, t_factor, t_solve, gemm_flops, getrf_flops, getrs_flops, qr_flops,
  *               t_getrf, t_schur, t_compress, factor_bytes; vals[11] = peak HBM bytes. */
 int rcs_solver_stats(rcs_solver_t s, int* info, double* vals);
+/* The statistics the reference driver prints through AIFMM::A (test/sphere.cpp:217-245):
+ * out4 = getAvgLeafSize (T:490-501), getAvgMultipoles (T:531-551), findMemory and
+ * findMemoryAfterFactorize (T:4511-4571, stored entries x 8 bytes in GB) of the current state. */
+int rcs_solver_tree_summary(rcs_solver_t s, double* out4);
 /* Per-box NumMultipoles, NumLocals, particle count at a level (3 ints per box). Returns
  * the number of boxes written, or a negative status. */
 int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap);

@@ -7,7 +7,8 @@
  *   ElectrostaticsMatrix include/ElectrostaticsMatrix.hpp:8-40
  *   LowRank<K>        include/ACA.hpp:7-30 (ACA_only_nodes2)
  *   RRQR              include/rrqr.hpp:5-27
- *   AIFMM<K>          include/AIFMM.hpp:11-36 (factorize, solve, backSubstitute, backSubstitute1/2/3, getPhi)
+ *   AIFMM<K>          include/AIFMM.hpp:11-36 (factorize, solve, backSubstitute, backSubstitute1/2/3, getPhi,
+ *                     and A for the tree statistics the driver prints)
  *
  * Header only, over the C ABI of rcs_amd.h (link with -lrcs_amd). Matrices are
  * column-major complex (rcs::Mat); the reference's Eigen Mat/Vec map onto data()/rows()/
@@ -21,6 +22,7 @@
 #define RCS_AMD_HPP
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <fstream>
@@ -416,6 +418,26 @@ class RRQR {
   }
 };
 
+// What the reference driver reads from the tree through AIFMM::A (test/sphere.cpp:217-245):
+// getAvgLeafSize (T:490-501), getAvgMultipoles (T:531-551), findMemory and
+// findMemoryAfterFactorize (T:4511-4571), each of the solver's current state.
+class TreeStatistics {
+ public:
+  explicit TreeStatistics(rcs_solver_t s) : s_(s) {}
+  int getAvgLeafSize() { return (int)summary()[0]; }
+  int getAvgMultipoles() { return (int)summary()[1]; }
+  double findMemory() { return summary()[2]; }
+  double findMemoryAfterFactorize() { return summary()[3]; }
+
+ private:
+  std::array<double, 4> summary() {
+    std::array<double, 4> out{};
+    check(rcs_solver_tree_summary(s_, out.data()));
+    return out;
+  }
+  rcs_solver_t s_;
+};
+
 // AIFMM<K> (include/AIFMM.hpp): construction assembles on the GPU, factorize() runs the
 // elimination, solve() returns the solution in box order (src/AIFMM.cpp:103-110) and
 // getPhi() the solution in the input order (src/AIFMM.cpp:135-147) of the last solve.
@@ -426,8 +448,12 @@ class AIFMM {
     const double bbox[6] = {in.xmin, in.xmax, in.ymin, in.ymax, in.zmin, in.zmax};
     check(rcs_solver_create(mykernel->handle(), bbox, in.leafSize, in.tol_pow, in.fillin_tol, in.fillin_rank,
                             precision, 0, &s_));
+    A = new TreeStatistics(s_);
   }
-  ~AIFMM() { if (s_) rcs_solver_destroy(s_); }
+  ~AIFMM() {
+    delete A;
+    if (s_) rcs_solver_destroy(s_);
+  }
   AIFMM(const AIFMM&) = delete;
   AIFMM& operator=(const AIFMM&) = delete;
   void factorize() { check(rcs_solver_factorize(s_)); }
@@ -460,6 +486,7 @@ class AIFMM {
   void getPhi(Vec& phi) { phi = phi_; }
   rcs_solver_t handle() const { return s_; }
   kerneltype* K;
+  TreeStatistics* A = nullptr;  // the reference's FMM2DTree* A, for the statistics drivers print
 
  private:
   rcs_solver_t s_ = nullptr;

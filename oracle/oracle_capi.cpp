@@ -364,6 +364,15 @@ int orc_aifmm_info(void* h, int* info, double* times, double* flops) {
   flops[0] = t->ctr.gemm_flops; flops[1] = t->ctr.getrf_flops; flops[2] = t->ctr.getrs_flops; flops[3] = t->ctr.qr_flops;
   return 0;
 }
+// getAvgLeafSize, getAvgMultipoles, findMemory, findMemoryAfterFactorize of the current tree.
+int orc_aifmm_tree_summary(void* h, double* out4) {
+  Tree* t = ((OSolver*)h)->S->A;
+  out4[0] = t->getAvgLeafSize();
+  out4[1] = t->getAvgMultipoles();
+  out4[2] = t->findMemory();
+  out4[3] = t->findMemoryAfterFactorize();
+  return 0;
+}
 // Per-box ranks at level j: NumMultipoles, NumLocals, particle count (3*nBoxes ints).
 int orc_aifmm_ranks(void* h, int j, int* out) {
   Tree* t = ((OSolver*)h)->S->A;

@@ -99,6 +99,50 @@ struct Tree {
 
   Box& B(int j, int k) { return tree[j][k]; }
 
+  // getAvgLeafSize T:490-501 (leaf boxes holding points), getAvgMultipoles T:531-551 (boxes
+  // with P2M rows and an interaction list, levels nLevels..startLevel), integer averages.
+  int getAvgLeafSize() const {
+    long long avg = 0, n = 0;
+    for (const Box& b : tree[nLevels])
+      if (!b.chargeLocations.empty()) { ++n; avg += (long long)b.chargeLocations.size(); }
+    return n ? (int)(avg / n) : 0;
+  }
+  int getAvgMultipoles() const {
+    long long avg = 0, n = 0;
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box& b : tree[j])
+        if (b.P2M.rows() > 0 && !b.interactionList.empty()) { ++n; avg += b.P2M.rows(); }
+    return n ? (int)(avg / n) : 0;
+  }
+  static double entriesOf(const std::map<int, Mat>& m, int key) {
+    auto it = m.find(key);
+    return it == m.end() ? 0.0 : (double)it->second.size();
+  }
+  // findMemory T:4511-4539: L2P and P2M of every box, M2L over interaction lists, P2P over
+  // neighbour lists at the leaf level; entries x 8 bytes, in GB.
+  double findMemory() const {
+    double sum = 0;
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box& b : tree[j]) sum += (double)b.L2P.size() + (double)b.P2M.size();
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box& b : tree[j])
+        for (int n : b.interactionList) sum += entriesOf(b.M2L, n);
+    for (const Box& b : tree[nLevels])
+      for (int n : b.neighborList) sum += entriesOf(b.P2P, n);
+    return sum * 8 * 1e-9;
+  }
+  // findMemoryAfterFactorize T:4541-4571: L2P and P2M, and P2P over neighbour lists at every
+  // level (the interaction-list part is commented out there).
+  double findMemoryAfterFactorize() const {
+    double sum = 0;
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box& b : tree[j]) sum += (double)b.L2P.size() + (double)b.P2M.size();
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box& b : tree[j])
+        for (int n : b.neighborList) sum += entriesOf(b.P2P, n);
+    return sum * 8 * 1e-9;
+  }
+
   void createTree() {  // T:167-192
     tree.assign(nLevels + 1, std::vector<Box>());
     for (int j = 0; j <= nLevels; ++j) tree[j].resize(nBoxesPerLevel[j]);

@@ -53,6 +53,7 @@ def lib():
         L.orc_aifmm_solve.argtypes = [vp, dp, dp, dp]
         L.orc_aifmm_info.argtypes = [vp, ip, dp, dp]
         L.orc_aifmm_ranks.argtypes = [vp, C.c_int, ip]
+        L.orc_aifmm_tree_summary.argtypes = [vp, dp]
         _LIB = L
     return _LIB
 
@@ -262,6 +263,13 @@ class AIFMM:
         out = np.zeros(3 * 8 ** level, dtype=np.int32)
         n = lib().orc_aifmm_ranks(self.h, level, _ip(out))
         return out[: 3 * n].reshape(n, 3)
+
+    def tree_summary(self):
+        """getAvgLeafSize, getAvgMultipoles, findMemory, findMemoryAfterFactorize (GB)."""
+        out = np.zeros(4)
+        lib().orc_aifmm_tree_summary(self.h, _dp(out))
+        return {"avg_leaf_size": int(out[0]), "avg_multipoles": int(out[1]), "memory_gb": out[2],
+                "memory_after_factorize_gb": out[3]}
 
 
 def panel(tc, a, b, c):

@@ -58,6 +58,7 @@ def lib():
         L.rcs_solver_solve.argtypes = [vp, dp, C.c_int, dp, dp]
         L.rcs_solver_stats.argtypes = [vp, ip, dp]
         L.rcs_solver_ranks.argtypes = [vp, C.c_int, ip, C.c_int]
+        L.rcs_solver_tree_summary.argtypes = [vp, dp]
         L.rcs_profile_enable.argtypes = [C.c_int]
         L.rcs_profile_get.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_longlong)]
         L.rcs_operator_create.argtypes = [vp, dp, C.c_int, C.c_int, C.POINTER(vp)]
@@ -86,7 +87,7 @@ EXPORTED = [
     "rcs_mesh_locations", "rcs_mesh_rwg", "rcs_mesh_npatch", "rcs_mesh_patches", "rcs_entries", "rcs_rhs", "rcs_rcs",
     "rcs_j_at_centroids", "rcs_rcs_from_j", "rcs_aca", "rcs_rrqr", "rcs_lu_solve", "rcs_gemm",
     "rcs_solver_create", "rcs_solver_destroy", "rcs_solver_factorize", "rcs_solver_solve", "rcs_solver_stats",
-    "rcs_solver_ranks", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
+    "rcs_solver_ranks", "rcs_solver_tree_summary", "rcs_solver_block", "rcs_profile_enable", "rcs_profile_reset", "rcs_profile_get",
     "rcs_operator_create", "rcs_operator_destroy", "rcs_operator_apply", "rcs_solver_refine",
     "rcs_mlp_create", "rcs_mlp_destroy", "rcs_mlp_get", "rcs_mlp_set", "rcs_mlp_forward", "rcs_mlp_gradient",
     "rcs_mlp_train", "rcs_set_rank_model", "rcs_set_aca_burst", "rcs_set_lane", "rcs_solver_rank_log", "rcs_solver_aca_stats",
@@ -318,6 +319,14 @@ class AIFMM:
              "n_compress_p2l": int(i[4])}
         d.update({k: float(x) for k, x in zip(keys, v)})
         return d
+
+    def tree_summary(self):
+        """getAvgLeafSize, getAvgMultipoles, findMemory, findMemoryAfterFactorize (GB) of the
+        current state, as the reference driver prints them."""
+        out = np.zeros(4)
+        _check(lib().rcs_solver_tree_summary(self.h, _dp(out)))
+        return {"avg_leaf_size": int(out[0]), "avg_multipoles": int(out[1]), "memory_gb": out[2],
+                "memory_after_factorize_gb": out[3]}
 
     def ranks(self, level):
         cap = 3 * 8 ** level

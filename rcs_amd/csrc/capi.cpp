@@ -737,6 +737,13 @@ int rcs_profile_get(int tag, double* work, double* ms, long long* launches) {
   return 0;
   CATCH
 }
+int rcs_solver_tree_summary(rcs_solver_t s, double* out4) {
+  TRY need(s, "solver");
+  need(out4, "out4");
+  s->s->tree_summary(out4);
+  return 0;
+  CATCH
+}
 int rcs_solver_ranks(rcs_solver_t s, int level, int* out, int cap) {
   TRY need(s, "solver");
   std::vector<int> v;

@@ -1714,6 +1714,37 @@ class Solver : public SolverBase {
     }
     return true;
   }
+  // getAvgLeafSize (T:490-501), getAvgMultipoles (T:531-551; integer averages as there) and
+  // findMemory / findMemoryAfterFactorize (T:4511-4571: stored matrix entries x 8 bytes in
+  // GB) from the shapes of the blocks held now.
+  void tree_summary(double* out) const override {
+    long long sum = 0, nb = 0;
+    for (const Box<T>& b : tree[nLevels])
+      if (!b.charge.empty()) { ++nb; sum += (long long)b.charge.size(); }
+    out[0] = nb ? (double)(sum / nb) : 0.0;
+    sum = 0;
+    nb = 0;
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box<T>& b : tree[j])
+        if (b.P2M.r > 0 && !b.IL.empty()) { ++nb; sum += b.P2M.r; }
+    out[1] = nb ? (double)(sum / nb) : 0.0;
+    auto entries = [](const std::map<int, DMat<T>>& m, int key) {
+      auto it = m.find(key);
+      return it == m.end() ? 0.0 : (double)it->second.size();
+    };
+    double bases = 0, far = 0, near_leaf = 0, near_all = 0;
+    for (int j = nLevels; j >= startLevel; --j)
+      for (const Box<T>& b : tree[j]) {
+        bases += (double)b.L2P.size() + (double)b.P2M.size();
+        for (int n : b.IL) far += entries(b.M2L, n);
+        double nearb = 0;
+        for (int n : b.NB) nearb += entries(b.P2P, n);
+        near_all += nearb;
+        if (j == nLevels) near_leaf += nearb;
+      }
+    out[2] = (bases + far + near_leaf) * 8 * 1e-9;
+    out[3] = (bases + near_all) * 8 * 1e-9;
+  }
   void ranks(int level, std::vector<int>& out) override {
     out.clear();
     if (level < 0 || level > nLevels) return;

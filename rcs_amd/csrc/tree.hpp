@@ -78,6 +78,9 @@ class SolverBase {
   virtual const std::vector<float>& rank_log() const = 0;
   virtual SolverStats stats() const = 0;
   virtual void ranks(int level, std::vector<int>& out) = 0;
+  // The tree statistics the reference driver prints: getAvgLeafSize, getAvgMultipoles,
+  // findMemory and findMemoryAfterFactorize of the current state, in the reference's units.
+  virtual void tree_summary(double* out4) const = 0;
   // Copy one stored block to the host as complex128 (which: 0 L2P, 1 P2M, 2 M2L[key],
   // 3 P2P[key], 4 M2P[key], 5 P2L[key], 6 L2M_f). Returns false if absent.
   virtual bool block(int level, int box, int which, int key, int& r, int& c, std::vector<double>& out) = 0;

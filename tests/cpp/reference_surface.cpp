@@ -55,7 +55,11 @@ int main(int argc, char** argv) {
 
   // S:205-264: AIFMM, factorize, the three-part solve, getPhi
   AIFMM<EFIEMatrix>* aifmm = new AIFMM<EFIEMatrix>(inputsToSolver, mykernel);
+  int avgLeafSize = aifmm->A->getAvgLeafSize();          // S:217
+  int avgRank = aifmm->A->getAvgMultipoles();            // S:223
+  double mem1 = 2 * aifmm->A->findMemory();              // S:224
   aifmm->factorize();
+  double mem2 = 2 * aifmm->A->findMemoryAfterFactorize();  // S:244
   aifmm->backSubstitute1(rhs);
   aifmm->backSubstitute2();
   aifmm->backSubstitute3();
@@ -106,7 +110,8 @@ int main(int argc, char** argv) {
   double area = lap->patchList[0].area;
 
   std::cout << numConductors << L << tp << rcs << e << r1.size() << c1.size() << r2.size() << c2.size() << m2.rows()
-            << m3.cols() << p.area << Q.cols() << maxPivot << err << C << area << xbox.size() << coefficients.size()
+            << m3.cols() << p.area << Q.cols() << maxPivot << err << C << area << xbox.size() << coefficients.size() << avgLeafSize << avgRank
+            << mem1 << mem2
             << std::endl;
   delete lap;
   delete aifmm;

@@ -450,3 +450,28 @@ def test_lanes_solve_independent_problems_concurrently():
     assert not err, err
     for (x0, xr0), (x1, xr1) in zip(ref, out):
         assert _rel(x1, x0) < 1e-12 and _rel(xr1, xr0) < 1e-10
+
+
+@pytest.mark.parametrize("kind", [R.LAPLACE, R.EFIE])
+def test_tree_statistics_match_oracle(kind):
+    """The statistics the reference driver prints through AIFMM::A, after assembly and after
+    elimination, on the elongated ellipsoid at 1 GHz TOL 5 (identical ranks on both sides,
+    so every stored block has the same shape)."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    g = R.Mesh(nodes, tris, kind, 1e9)
+    o = O.Mesh(nodes, tris)
+    loc = g.locations()
+    bb = M.bounding_box(loc)
+    sg = R.AIFMM(g, bb, 64, 5, 5)
+    so = O.AIFMM(o, O.EFIE if kind == R.EFIE else O.LAPLACE, 1e9, loc, bb, 64, 5, 5)
+    def same(stage):
+        a, b = sg.tree_summary(), so.tree_summary()
+        assert (a["avg_leaf_size"], a["avg_multipoles"]) == (b["avg_leaf_size"], b["avg_multipoles"]), stage
+        assert abs(a["memory_gb"] - b["memory_gb"]) <= 1e-12 * b["memory_gb"], stage
+        assert abs(a["memory_after_factorize_gb"] - b["memory_after_factorize_gb"]) <= \
+            1e-12 * b["memory_after_factorize_gb"], stage
+
+    same("assembled")
+    sg.factorize()
+    so.factorize()
+    same("factorized")

@@ -396,3 +396,27 @@ def test_reference_edge_map_lookup_cost_runs(fixture78):
     m = O.Mesh(nodes, tris)
     walk, entry = m.map_walk_cost(1.0e9, 200, 3)
     assert walk > 0 and entry > 0
+
+
+def test_tree_statistics_cross_checks():
+    """getAvgLeafSize / getAvgMultipoles / findMemory / findMemoryAfterFactorize
+    (T:490-551, 4511-4571) against other views of the same tree: the leaf average equals the
+    integer mean of the point counts that ranks() reports; before elimination every P2M has
+    NumMultipoles rows, so the multipole average equals the mean of NumMultipoles; and before
+    elimination the two memory figures differ exactly by the interaction-list M2L blocks,
+    which findMemory counts and the other does not, so the first is the larger."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    o = O.Mesh(nodes, tris)
+    loc = o.rwg_midpoints()
+    so = O.AIFMM(o, O.EFIE, 1e9, loc, M.bounding_box(loc), 64, 5, 5)
+    inf = so.info()
+    s0 = so.tree_summary()
+    counts = so.ranks(inf["nLevels"])[:, 2]
+    assert s0["avg_leaf_size"] == counts[counts > 0].sum() // (counts > 0).sum()
+    nm = np.concatenate([so.ranks(j)[:, 0] for j in range(inf["startLevel"], inf["nLevels"] + 1)])
+    assert s0["avg_multipoles"] == nm[nm > 0].sum() // (nm > 0).sum()
+    assert s0["memory_gb"] > s0["memory_after_factorize_gb"] > 0
+    so.factorize()
+    s1 = so.tree_summary()
+    assert s1["avg_leaf_size"] == s0["avg_leaf_size"]
+    assert s1["memory_after_factorize_gb"] >= s0["memory_after_factorize_gb"]  # fill-ins only add blocks
