@@ -121,6 +121,26 @@ int orc_rcs(void* h, double freq, const double* coef, const double* theta, const
   ORC_CATCH
 }
 
+// The EFIE integration primitives over source patch `tri` seen from the point tc:
+// getScalarIntegrationGreensFunction (E:670-685) -> s[2] and
+// getVectorIntegrationGreensFunction (E:386-446), the integral weighted by rho = r' - node fv,
+// -> v[6] (x, y, z complex). kind 2 projects onto the patch's own plane (test-only).
+int orc_efie_integrals(void* h, int kind, double freq, const double* tc, int tri, int fv, double* s, double* v) {
+  ORC_TRY
+  std::unique_ptr<Kernel> K(makeKernel((OMesh*)h, kind, freq));
+  const EFIE* e = dynamic_cast<const EFIE*>(K.get());
+  if (!e) throw std::runtime_error("EFIE kernel expected");
+  const Mesh& m = ((OMesh*)h)->mesh;
+  if (tri < 0 || tri >= (int)m.patches.size()) throw std::runtime_error("patch out of range");
+  V3 t(tc[0], tc[1], tc[2]);
+  cd a = e->scalarIntegration(t, tri, fv);
+  C3 b = e->vectorIntegration(t, tri, fv);
+  s[0] = a.real(); s[1] = a.imag();
+  v[0] = b.x.real(); v[1] = b.x.imag(); v[2] = b.y.real(); v[3] = b.y.imag(); v[4] = b.z.real(); v[5] = b.z.imag();
+  return 0;
+  ORC_CATCH
+}
+
 // getJAtCentroids (E:1100-1130): J is n_patch x 3 complex, column-major.
 int orc_j_at_centroids(void* h, double freq, const double* coef, double* J) {
   ORC_TRY

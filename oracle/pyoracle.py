@@ -37,6 +37,7 @@ def lib():
         L.orc_rhs.argtypes = [vp, C.c_double, dp, dp, dp]
         L.orc_rcs.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int, dp]
         L.orc_j_at_centroids.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_efie_integrals.argtypes = [vp, C.c_int, C.c_double, dp, C.c_int, C.c_int, dp, dp]
         L.orc_aca.argtypes = [vp, C.c_int, C.c_double, ip, C.c_int, ip, C.c_int, C.c_double, ip, ip, ip]
         L.orc_aca_dense.argtypes = [dp, C.c_int, C.c_int, C.c_double, ip, ip, ip]
         L.orc_map_walk_cost.argtypes = [vp, C.c_double, C.c_int, C.c_uint, dp, dp]
@@ -133,6 +134,15 @@ class Mesh:
         _check(lib().orc_rcs(self.h, freq, coef.ctypes.data_as(C.POINTER(C.c_double)), _dp(theta), _dp(phi),
                              theta.size, _dp(out)))
         return out
+
+    def efie_integrals(self, kind, freq, tc, tri, fv):
+        """(scalar, vector) Green's-function integrals over patch tri seen from tc, the vector
+        one weighted by r' - node fv (E:670-685, E:386-446)."""
+        tc = np.ascontiguousarray(tc, dtype=np.float64)
+        s = np.zeros(2)
+        v = np.zeros(6)
+        _check(lib().orc_efie_integrals(self.h, kind, freq, _dp(tc), tri, fv, _dp(s), _dp(v)))
+        return s[0] + 1j * s[1], v[0::2] + 1j * v[1::2]
 
     def j_at_centroids(self, freq, coef):
         coef = np.ascontiguousarray(coef, dtype=np.complex128)

@@ -77,6 +77,40 @@ def test_panel_integral_in_plane_point():
     assert abs(lap - ref) < 1e-5 * ref
 
 
+@pytest.mark.parametrize("d,kind,tol", [(1.5, O.EFIE_PHYSICAL_PROJECTION, 1e-5), (3.0, O.EFIE_PHYSICAL_PROJECTION, 1e-5),
+                                         (6.0, O.EFIE, 1e-6), (9.0, O.EFIE, 1e-6),
+                                         (15.0, O.EFIE, 1e-2), (40.0, O.EFIE, 1e-2)])
+def test_efie_integration_primitives_vs_quadrature(d, kind, tol):
+    """getScalarIntegrationGreensFunction (E:670-685) and getVectorIntegrationGreensFunction
+    (E:386-446): the integrals of G = -exp(-ikR)/(4 pi R) and of G (r' - r_free) over a source
+    patch, against brute-force subdivided quadrature, at observation points d largest edges
+    above the patch: the analytic singular-extraction branch (d < 4; projection on the patch's
+    own plane, since the reference's origin-plane projection is not the exact integral), the
+    7-point branch (4 <= d < 10; bounded by the truncated constants of E:284-286) and the
+    1-point branch (d >= 10; bounded by the one-point rule itself). A wrong 7-point weight or
+    point, free vertex, sign or wavenumber fails the first two bars by orders of magnitude."""
+    freq = 1e9
+    k = 2 * np.pi * freq / C0
+    nodes, tris = M.icosphere(2, 0.15)
+    o = O.Mesh(nodes, tris)
+    full = np.vstack([np.zeros((1, 3)), nodes])
+    P = o.patches()
+    for tri in (0, 37, 101):
+        v = tris[tri]
+        a, b, c = full[v[0]], full[v[1]], full[v[2]]
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        tc = P[tri, :3] + n * d * P[tri, 4] + (b - a) * 0.13
+        R = lambda x: np.linalg.norm(x - tc, axis=1)
+        G = lambda x: (-np.exp(-1j * k * R(x)) / (4 * np.pi * R(x)))[:, None]
+        ref_s = _quad_triangle(G, a, b, c, depth=7)[0]
+        for fv in v:
+            ref_v = _quad_triangle(lambda x: G(x) * (x - full[fv]), a, b, c, depth=7)
+            s, vec = o.efie_integrals(kind, freq, tc, tri, int(fv))
+            assert abs(s - ref_s) < tol * abs(ref_s), (tri, fv)
+            assert np.linalg.norm(vec - ref_v) < tol * np.linalg.norm(ref_v), (tri, fv)
+
+
 def test_laplace_capacitance_sphere():
     """Sphere of radius a: C = 4 pi eps0 a = 16.69 pF (a = 0.15 m), faceting error ~1%."""
     nodes, tris = M.icosphere(3, 0.15)
