@@ -111,6 +111,29 @@ def test_efie_integration_primitives_vs_quadrature(d, kind, tol):
             assert np.linalg.norm(vec - ref_v) < tol * np.linalg.norm(ref_v), (tri, fv)
 
 
+@pytest.mark.parametrize("lo,hi,tol", [(1.0, 4.0, 1e-5), (4.0, 10.0, 1e-6), (10.0, 30.0, 1e-2)])
+def test_laplace_entries_vs_quadrature(lo, hi, tol):
+    """ElectrostaticsMatrix::getMatrixEntry (src/ElectrostaticsMatrix.cpp:293-340):
+    C0 * integral of 1/|c_i - r'| over patch j, C0 = 1/(4 pi eps0), against brute-force
+    quadrature for patch pairs whose centroid distance lies in [lo, hi) largest edges of
+    patch j: the analytic branch, the 7-point branch and the one-point branch."""
+    nodes, tris = M.icosphere(3, 0.15)
+    o = O.Mesh(nodes, tris)
+    full = np.vstack([np.zeros((1, 3)), nodes])
+    P = o.patches()
+    C0 = 1.0 / (4 * np.pi * EPS0)
+    cen, le = P[:, :3], P[:, 4]
+    D = np.linalg.norm(cen[:, None, :] - cen[None, :, :], axis=2) / le[None, :]
+    pairs = np.argwhere((D >= lo) & (D < hi))[::97][:8]
+    assert len(pairs) >= 4
+    Z = o.entries(O.LAPLACE, 1e9, pairs[:, 0], pairs[:, 1])
+    for q, (i, j) in enumerate(pairs):
+        v = tris[j]
+        ref = C0 * _quad_triangle(lambda x: (1.0 / np.linalg.norm(x - cen[i], axis=1))[:, None],
+                                  full[v[0]], full[v[1]], full[v[2]], depth=6)[0]
+        assert abs(Z[q, q].real - ref) < tol * ref and Z[q, q].imag == 0, (i, j)
+
+
 def test_laplace_capacitance_sphere():
     """Sphere of radius a: C = 4 pi eps0 a = 16.69 pF (a = 0.15 m), faceting error ~1%."""
     nodes, tris = M.icosphere(3, 0.15)
