@@ -151,21 +151,45 @@ std::unique_ptr<rcs_mesh_s> build_mesh(const double* nodes, int n_nodes, const i
   return m;
 }
 
+// Pool blocks of one C-ABI call, returned to the pool when the call ends, also when it fails
+// half way. Their reuse is ordered after their last use because all work runs on the
+// library's one stream.
+struct Blocks {
+  DevPool& pool;
+  std::vector<void*> held;
+  explicit Blocks(DevPool& p) : pool(p) {}
+  Blocks(const Blocks&) = delete;
+  Blocks& operator=(const Blocks&) = delete;
+  void* alloc(size_t bytes) {
+    void* q = pool.alloc(bytes);
+    held.push_back(q);
+    return q;
+  }
+  ~Blocks() {
+    for (void* q : held) {
+      try {
+        pool.free(q);
+      } catch (...) {
+      }
+    }
+  }
+};
+
 template <typename T>
-DMat<T> dalloc(Ctx& cx, int r, int c) {
+DMat<T> dalloc(Blocks& b, int r, int c) {
   DMat<T> m;
   m.r = r;
   m.c = c;
   m.ld = std::max(r, 1);
-  m.p = (T*)cx.pool.alloc(sizeof(T) * std::max<size_t>((size_t)r * c, 1));
+  m.p = (T*)b.alloc(sizeof(T) * std::max<size_t>((size_t)r * c, 1));
   return m;
 }
 // host complex128 -> device T
 template <typename T>
-DMat<T> upload(Ctx& cx, const double* h, int r, int c, int ldh);
+DMat<T> upload(Ctx& cx, Blocks& b, const double* h, int r, int c, int ldh);
 template <>
-DMat<cpx<double>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
-  DMat<cpx<double>> m = dalloc<cpx<double>>(cx, r, c);
+DMat<cpx<double>> upload(Ctx& cx, Blocks& b, const double* h, int r, int c, int ldh) {
+  DMat<cpx<double>> m = dalloc<cpx<double>>(b, r, c);
   if (ldh == m.ld) {
     cx.upload(m.p, h, sizeof(cpx<double>) * (size_t)r * c);
   } else {
@@ -175,34 +199,32 @@ DMat<cpx<double>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
   return m;
 }
 template <>
-DMat<cpx<float>> upload(Ctx& cx, const double* h, int r, int c, int ldh) {
-  DMat<cpx<double>> t = upload<cpx<double>>(cx, h, r, c, ldh);
-  DMat<cpx<float>> m = dalloc<cpx<float>>(cx, r, c);
+DMat<cpx<float>> upload(Ctx& cx, Blocks& b, const double* h, int r, int c, int ldh) {
+  DMat<cpx<double>> t = upload<cpx<double>>(cx, b, h, r, c, ldh);
+  DMat<cpx<float>> m = dalloc<cpx<float>>(b, r, c);
   cast_c128_to_c64(m, t, cx.st);
-  cx.sync();
-  cx.pool.free(t.p);
   return m;
 }
 template <typename T>
-void download(Ctx& cx, DMat<T> m, double* h);
+void download(Ctx& cx, Blocks& b, DMat<T> m, double* h);
 template <>
-void download(Ctx& cx, DMat<cpx<double>> m, double* h) {
+void download(Ctx& cx, Blocks&, DMat<cpx<double>> m, double* h) {
   HIPCHK(hipMemcpy2DAsync(h, sizeof(cpx<double>) * m.r, m.p, sizeof(cpx<double>) * m.ld, sizeof(cpx<double>) * m.r,
                           m.c, hipMemcpyDeviceToHost, cx.st));
   cx.sync();
 }
 template <>
-void download(Ctx& cx, DMat<cpx<float>> m, double* h) {
-  DMat<cpx<double>> t = dalloc<cpx<double>>(cx, m.r, m.c);
+void download(Ctx& cx, Blocks& b, DMat<cpx<float>> m, double* h) {
+  DMat<cpx<double>> t = dalloc<cpx<double>>(b, m.r, m.c);
   cast_c64_to_c128(t, m, cx.st);
-  download<cpx<double>>(cx, t, h);
-  cx.pool.free(t.p);
+  download<cpx<double>>(cx, b, t, h);
 }
 
 template <typename T>
 void entries_t(rcs_mesh_t m, const int* rows, int nr, const int* cols, int nc, double* out) {
   Ctx& cx = ctx();
-  DMat<T> Z = dalloc<T>(cx, nr, nc);
+  Blocks b(cx.pool);
+  DMat<T> Z = dalloc<T>(b, nr, nc);
   if (nr > 0 && nc > 0) {
     EntryTask<T> t{};
     t.rows = (const int*)cx.push(rows, sizeof(int) * nr);
@@ -214,9 +236,8 @@ void entries_t(rcs_mesh_t m, const int* rows, int nr, const int* cols, int nc, d
     t.tile0 = 0;
     const EntryTask<T>* d = (const EntryTask<T>*)cx.push(&t, sizeof(t));
     launch_entries<T>(m->g->d, m->g->h.kind, d, 1, entry_tiles(nr, nc), cx.st);
-    download<T>(cx, Z, out);
+    download<T>(cx, b, Z, out);
   }
-  cx.pool.free(Z.p);
 }
 
 template <typename T>
@@ -239,39 +260,37 @@ template <typename T>
 void rrqr_t(const double* A, int m, int n, int tp1, int tp2, int partition, double ip, int fr, double* Q, int* rank,
             double* mp) {
   Ctx& cx = ctx();
-  DMat<T> dA = upload<T>(cx, A, m, n, m);
+  Blocks b(cx.pool);
+  DMat<T> dA = upload<T>(cx, b, A, m, n, m);
   int mn = std::max(1, std::min(m, n));
-  DMat<T> dQ = dalloc<T>(cx, m, mn);
+  DMat<T> dQ = dalloc<T>(b, m, mn);
   RRQRInfo inf = rrqr(dA, partition, std::pow(10.0, -tp1), std::pow(10.0, -tp2), ip, fr, dQ, cx);
   *rank = inf.rank;
   *mp = inf.maxPivot;
-  if (inf.rank > 0) download<T>(cx, dQ.cols(0, inf.rank), Q);
-  cx.pool.free(dA.p);
-  cx.pool.free(dQ.p);
+  if (inf.rank > 0) download<T>(cx, b, dQ.cols(0, inf.rank), Q);
 }
 
 template <typename T>
 void lu_t(const double* A, int n, const double* B, int nrhs, double* X) {
   Ctx& cx = ctx();
-  DMat<T> dA = upload<T>(cx, A, n, n, n);
-  DMat<T> dB = upload<T>(cx, B, n, nrhs, n);
-  int* ip = (int*)cx.pool.alloc(sizeof(int) * 2 * std::max(n, 1));  // pivots + permutation
+  Blocks b(cx.pool);
+  DMat<T> dA = upload<T>(cx, b, A, n, n, n);
+  DMat<T> dB = upload<T>(cx, b, B, n, nrhs, n);
+  int* ip = (int*)b.alloc(sizeof(int) * 2 * std::max(n, 1));  // pivots + permutation
   getrf(dA, ip, cx);
   getrs(dA, ip, dB, cx);
-  download<T>(cx, dB, X);
-  cx.pool.free(dA.p);
-  cx.pool.free(dB.p);
-  cx.pool.free(ip);
+  download<T>(cx, b, dB, X);
 }
 
 template <typename T>
 void gemm_t(int opA, int opB, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C) {
   Ctx& cx = ctx();
+  Blocks b(cx.pool);
   int ar = opA == OP_N ? m : k, ac = opA == OP_N ? k : m;
   int br = opB == OP_N ? k : n, bc = opB == OP_N ? n : k;
-  DMat<T> dA = upload<T>(cx, A, ar, ac, lda);
-  DMat<T> dB = upload<T>(cx, B, br, bc, ldb);
-  DMat<T> dC = dalloc<T>(cx, m, n);
+  DMat<T> dA = upload<T>(cx, b, A, ar, ac, lda);
+  DMat<T> dB = upload<T>(cx, b, B, br, bc, ldb);
+  DMat<T> dC = dalloc<T>(b, m, n);
   std::vector<GemmOp<T>> g(1);
   g[0] = GemmOp<T>{};
   g[0].A = dA.p; g[0].lda = dA.ld; g[0].opA = opA;
@@ -281,10 +300,7 @@ void gemm_t(int opA, int opB, int m, int n, int k, const double* A, int lda, con
   g[0].alpha = T(1, 0); g[0].beta = T(0, 0);
   if (k == 0) zero2d(dC, cx.st);
   else gemm_grouped(g, cx);
-  download<T>(cx, dC, C);
-  cx.pool.free(dA.p);
-  cx.pool.free(dB.p);
-  cx.pool.free(dC.p);
+  download<T>(cx, b, dC, C);
 }
 }  // namespace
 
@@ -649,15 +665,14 @@ int rcs_operator_apply(rcs_operator_t o, const double* x, int nrhs, double* y) {
   need(x, "x");
   need(y, "y");
   Ctx& cx = o->s->context();
+  Blocks b(cx.pool);
   const size_t bytes = sizeof(cpx<double>) * (size_t)o->s->size() * nrhs;
-  void* dx = cx.pool.alloc(bytes);
-  void* dy = cx.pool.alloc(bytes);
+  void* dx = b.alloc(bytes);
+  void* dy = b.alloc(bytes);
   cx.upload(dx, x, bytes);
   o->s->apply_device(dx, nrhs, dy);
   HIPCHK(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, cx.st));
   cx.sync();
-  cx.pool.free(dx);
-  cx.pool.free(dy);
   return 0;
   CATCH
 }
@@ -672,15 +687,14 @@ int rcs_solver_refine(rcs_solver_t s, rcs_operator_t o, const double* rhs, int n
   if (s->s->size() != o->s->size()) throw std::runtime_error("refine: operator and solver sizes differ");
   Ctx& cx = s->s->context();
   if (&o->s->context() != &cx) throw std::runtime_error("refine: operator and solver were created in different lanes");
+  Blocks b(cx.pool);
   const size_t bytes = sizeof(cpx<double>) * (size_t)s->s->size() * nrhs;
-  cpx<double>* db = (cpx<double>*)cx.pool.alloc(bytes);
-  cpx<double>* dx = (cpx<double>*)cx.pool.alloc(bytes);
+  cpx<double>* db = (cpx<double>*)b.alloc(bytes);
+  cpx<double>* dx = (cpx<double>*)b.alloc(bytes);
   cx.upload(db, rhs, bytes);
   RefineResult r = gmres_refine(*o->s, *s->s, db, nrhs, dx, tol, maxit, restart, cx);
   HIPCHK(hipMemcpyAsync(x, dx, bytes, hipMemcpyDeviceToHost, cx.st));
   cx.sync();
-  cx.pool.free(db);
-  cx.pool.free(dx);
   if (iters) *iters = r.iters;
   if (relres) *relres = r.relres;
   return 0;
