@@ -75,15 +75,25 @@ def perturbed_ellipsoid(level: int, axes, noise: float, seed: int):
     return nodes * scale, tris
 
 
-def write_reference_files(nodes, tris, node_path, patch_path):
+def write_reference_files(nodes, tris, node_path, patch_path, conductor=None):
+    """Node lines "x y z"; patch lines "i j k" (1-based), or "i j k c" with a conductor index."""
     np.savetxt(node_path, nodes, fmt="%.18e")
-    np.savetxt(patch_path, tris, fmt="%d")
+    t = np.asarray(tris)
+    if conductor is not None:
+        t = np.column_stack([t, np.asarray(conductor)])
+    np.savetxt(patch_path, t, fmt="%d")
 
 
-def read_reference_files(node_path, patch_path):
+def read_reference_files(node_path, patch_path, with_conductor=False):
+    """Inverse of write_reference_files. A patch file without the fourth column gives
+    conductor 0 for every patch, as the reference's reader does (src/EFIEMatrix.cpp:72-95)."""
     nodes = np.loadtxt(node_path, dtype=np.float64).reshape(-1, 3)
     tris = np.loadtxt(patch_path, dtype=np.int64).reshape(-1, np.loadtxt(patch_path, max_rows=1).size)
-    return nodes, tris[:, :3].astype(np.int32)
+    t3 = tris[:, :3].astype(np.int32)
+    if not with_conductor:
+        return nodes, t3
+    cond = tris[:, 3].astype(np.int32) if tris.shape[1] > 3 else np.zeros(len(tris), dtype=np.int32)
+    return nodes, t3, cond
 
 
 def bounding_box(points):

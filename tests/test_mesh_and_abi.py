@@ -31,6 +31,14 @@ def test_reference_format_roundtrip(tmp_path):
     M.write_reference_files(nodes, tris, n, p)
     n2, t2 = M.read_reference_files(n, p)
     assert np.array_equal(t2, tris) and np.allclose(n2, nodes, rtol=0, atol=1e-17)
+    assert np.array_equal(M.read_reference_files(n, p, with_conductor=True)[2], np.zeros(len(tris)))
+    cond = np.arange(len(tris)) % 2
+    M.write_reference_files(nodes, tris, n, p, conductor=cond)
+    n3, t3, c3 = M.read_reference_files(n, p, with_conductor=True)
+    assert np.array_equal(t3, tris) and np.array_equal(c3, cond)
+    # the C++ reader of include/rcs_amd.hpp parses the same lines (conductor optional)
+    lines = [ln.split() for ln in open(p)]
+    assert all(len(ln) == 4 for ln in lines)
 
 
 def test_perturbed_ellipsoid_seeded():
