@@ -36,9 +36,24 @@ def test_reference_format_roundtrip(tmp_path):
     M.write_reference_files(nodes, tris, n, p, conductor=cond)
     n3, t3, c3 = M.read_reference_files(n, p, with_conductor=True)
     assert np.array_equal(t3, tris) and np.array_equal(c3, cond)
-    # the C++ reader of include/rcs_amd.hpp parses the same lines (conductor optional)
-    lines = [ln.split() for ln in open(p)]
-    assert all(len(ln) == 4 for ln in lines)
+    # the C++ reader of include/rcs_amd.hpp (host only, no library needed) reads the same
+    # files: with and without the conductor column
+    import subprocess
+    src = tmp_path / "read.cpp"
+    src.write_text('#include "rcs_amd.hpp"\n#include <iostream>\nint main(int, char** a) {\n'
+                   '  std::vector<double> x; std::vector<int> t, c;\n'
+                   '  rcs::readMeshFiles(a[1], a[2], x, t, &c);\n'
+                   '  long s = 0; for (int v : c) s += v; long st = 0; for (int v : t) st += v;\n'
+                   '  std::cout << x.size() / 3 << " " << t.size() / 3 << " " << c.size() << " " << s << " " << st;\n}\n')
+    exe = tmp_path / "read"
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for conductor in (cond, None):
+        M.write_reference_files(nodes, tris, n, p, conductor=conductor)
+        out = subprocess.run([str(exe), str(n), str(p)], capture_output=True, text=True, timeout=60).stdout.split()
+        want_c = 0 if conductor is None else int(cond.sum())
+        assert list(map(int, out)) == [len(nodes), len(tris), len(tris), want_c, int(tris.sum())]
 
 
 def test_perturbed_ellipsoid_seeded():
