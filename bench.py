@@ -78,6 +78,25 @@ def cpu_baseline(level_cpu, radius, freq, tol, gpu_flops, level_bench=None):
         "sample_wall_s": wall,
         "sample_t_assemble": inf["t_assemble"], "sample_t_factor": inf["t_factor"], "sample_t_solve": inf["t_solve"],
     }
+    # the same sample with the elimination's and solve's dense products, LU and solves on
+    # all cores (results are bit-identical; the reference driver runs them on one thread)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        old = O.set_dense_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            s2 = O.AIFMM(m, O.EFIE, freq, loc, M.bounding_box(loc), 64, tol, tol)
+            s2.factorize()
+            s2.solve(b)
+            wall2 = time.perf_counter() - t0
+        finally:
+            O.set_dense_threads(old)
+        inf2 = s2.info()
+        out["all_cores"] = {"value": 1.0 / (wall2 * gpu_flops / max(flops, 1.0)), "threads": threads,
+                            "sample_wall_s": wall2, "sample_t_assemble": inf2["t_assemble"],
+                            "sample_t_factor": inf2["t_factor"], "sample_t_solve": inf2["t_solve"]}
+    except Exception as e:
+        out["all_cores"] = {"error": repr(e)}
     if level_bench is not None:
         try:
             out["reference_map_walk"] = reference_map_walk(level_bench, radius, freq)

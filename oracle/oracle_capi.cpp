@@ -57,6 +57,13 @@ struct DenseMatrix : Kernel {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+// Threads of the oracle's dense products, LU and solves (the elimination and the solve);
+// assembly uses OpenMP's own default. Returns the previous count.
+int orc_set_dense_threads(int n) {
+  int old = dense_threads();
+  dense_threads() = n < 1 ? 1 : n;
+  return old;
+}
 
 void* orc_mesh_create(const double* nodes, int nNodes, const int* tris, int nTri, const int* conductor) {
   OMesh* m = new OMesh;

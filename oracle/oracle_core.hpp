@@ -102,10 +102,21 @@ inline Mat adjoint(const Mat& A) {
   return B;
 }
 
+// Threads of the dense products, LU and solves (1 by default: the reference driver runs
+// factorize and solve on one thread, test/sphere.cpp:206-207). Every parallel loop below is
+// over independent output columns, so results do not depend on the count.
+inline int& dense_threads() {
+  static int t = 1;
+  return t;
+}
+inline bool dense_parallel(double work) { return dense_threads() > 1 && work > 5.0e5; }
+
 // C = A*B (inner dimensions must agree, as Eigen asserts).
 inline Mat mul(const Mat& A, const Mat& B) {
   if (A.c != B.r) throw std::runtime_error("oracle: invalid matrix product");
   Mat C(A.r, B.c);
+#pragma omp parallel for schedule(static) num_threads(dense_threads()) \
+    if (dense_parallel((double)A.r * A.c * B.c))
   for (int j = 0; j < B.c; ++j) {
     cd* cc = C.col(j);
     for (int k = 0; k < A.c; ++k) {
@@ -165,6 +176,8 @@ struct LU {
       if (piv != cd(0)) {
         for (int i = k + 1; i < n; ++i) lu(i, k) /= piv;
       }
+#pragma omp parallel for schedule(static) num_threads(dense_threads()) \
+    if (dense_parallel((double)(n - k) * (n - k)))
       for (int j = k + 1; j < n; ++j) {
         cd f = lu(k, j);
         if (f == cd(0)) continue;
@@ -177,6 +190,8 @@ struct LU {
   Mat solve(const Mat& B) const {
     if (B.r != n) throw std::runtime_error("oracle: LU solve size mismatch");
     Mat X(n, B.c);
+#pragma omp parallel for schedule(static) num_threads(dense_threads()) \
+    if (dense_parallel((double)n * n * B.c))
     for (int j = 0; j < B.c; ++j) {
       cd* x = X.col(j);
       for (int i = 0; i < n; ++i) x[i] = B(perm[i], j);

@@ -26,6 +26,7 @@ def lib():
         L = C.CDLL(path)
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
         L.orc_last_error.restype = C.c_char_p
+        L.orc_set_dense_threads.argtypes = [C.c_int]
         L.orc_mesh_create.restype = vp
         L.orc_mesh_create.argtypes = [dp, C.c_int, ip, C.c_int, ip]
         L.orc_mesh_destroy.argtypes = [vp]
@@ -184,6 +185,12 @@ class Mesh:
         _check(lib().orc_matvec(self.h, kind, freq, x.ctypes.data_as(C.POINTER(C.c_double)),
                                 y.ctypes.data_as(C.POINTER(C.c_double))))
         return y
+
+
+def set_dense_threads(n):
+    """Threads of the dense products, LU and solves (1, the reference driver's policy, by
+    default); returns the previous count."""
+    return int(lib().orc_set_dense_threads(int(n)))
 
 
 def aca_dense(A, tol):

@@ -477,3 +477,22 @@ def test_tree_statistics_cross_checks():
     s1 = so.tree_summary()
     assert s1["avg_leaf_size"] == s0["avg_leaf_size"]
     assert s1["memory_after_factorize_gb"] >= s0["memory_after_factorize_gb"]  # fill-ins only add blocks
+
+
+def test_dense_threads_do_not_change_results(rng):
+    """The oracle's dense LU and solve run on one thread by default (the reference driver's
+    policy) and on several for the all-cores CPU baseline; their parallel loops are over
+    independent columns, so the results are bit-identical. n = 800 puts the LU's trailing
+    updates and the solve above the parallel threshold."""
+    n = 800
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    B = rng.normal(size=(n, 24)) + 1j * rng.normal(size=(n, 24))
+    old = O.set_dense_threads(1)
+    try:
+        X1 = O.lu_solve(A, B)
+        O.set_dense_threads(8)
+        X8 = O.lu_solve(A, B)
+    finally:
+        O.set_dense_threads(old)
+    assert np.array_equal(X1, X8)
+    assert np.linalg.norm(A @ X1 - B) < 1e-10 * np.linalg.norm(B)
