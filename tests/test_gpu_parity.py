@@ -475,3 +475,34 @@ def test_tree_statistics_match_oracle(kind):
     sg.factorize()
     so.factorize()
     same("factorized")
+
+
+def test_full_size_sphere_properties():
+    """The benchmark's 30,720-unknown EFIE sphere (icosphere level 5, r = 0.15 m, 1 GHz,
+    TOL 6), where the oracle is too slow to compare against, through size-independent
+    properties of the solve:
+    * linearity, exactly: the solution for 2b is 2x bit for bit (a power of two scales every
+      product, FMA and division exactly and changes no pivot choice; the solve's reductions
+      run in a fixed order);
+    * right-hand sides are independent: a two-column solve gives the two single solutions;
+    * the direct solution's relative residual against an FP64 TOL 10 operator sits at the
+      compression level (below 1e-2), and flexible GMRES preconditioned by the factorization
+      reaches 1e-6 within 10 iterations."""
+    nodes, tris = M.icosphere(5, 0.15)
+    g = R.Mesh(nodes, tris, R.EFIE, 1e9)
+    assert g.N == 30720
+    bb = M.bounding_box(g.locations())
+    b = g.rhs()
+    s = R.AIFMM(g, bb, 64, 6, 6)
+    s.factorize()
+    x = s.solve(b)
+    assert np.array_equal(s.solve(2 * b), 2 * x)
+    kap = 2 * np.pi * 1e9 / 3e8
+    b2 = g.rhs((0.0, 1.0, 0.0), (kap, 0.0, 0.0))
+    X = s.solve(np.column_stack([b, b2]))
+    assert _rel(X[:, 0], x) < 1e-13 and _rel(X[:, 1], s.solve(b2)) < 1e-13
+    op = R.Operator(g, bb, 64, 10)
+    rr0 = _rel(op.apply(x), b)
+    xr, it, rr = s.refine(op, b, tol=1e-6, maxit=30, restart=30)
+    print(f"direct relres {rr0:.2e}; refined {rr:.1e} after {it} iterations")
+    assert rr0 < 1e-2 and rr <= 1e-6 and it <= 10
