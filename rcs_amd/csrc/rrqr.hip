@@ -131,11 +131,11 @@ __global__ void __launch_bounds__(256) rrqr_norms_kernel(RRQRJobs<T> jobs, RRQRI
 #pragma unroll
     for (int t = 1; t < RQ_MAXJ; ++t)
       if (t < jobs.n && jobs.j[t].wg0 <= (int)blockIdx.x) q = t;
-    RRQRState s0 = init.s[0];
+    // stored straight from the argument block: a selected aggregate would be built in
+    // scratch memory
 #pragma unroll
-    for (int t = 1; t < RQ_MAXJ; ++t)
-      if (t == q) s0 = init.s[t];
-    jb.st[0] = s0;
+    for (int t = 0; t < RQ_MAXJ; ++t)
+      if (t == q) jb.st[0] = init.s[t];
   }
   const T* a = jb.A[0] + (size_t)c * jb.m;
   double s = 0;
@@ -193,8 +193,12 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
   const int j = s.j, m = s.m, n = s.n, buf = s.buf;
   const int mlo = sh_mlo, mhi = sh_mhi;  // norms of [mlo, mhi) read as zero in this step
   const int end = (s.phase == 1) ? s.partition : n;
-  const double* nsi = jb.ns[buf];
-  const T* Ain = jb.A[buf];
+  // buf is 0 or 1; selecting keeps the job's pointer pairs in registers (indexing them with
+  // a run-time value would place the whole job in scratch memory)
+  const double* nsi = buf ? jb.ns[1] : jb.ns[0];
+  const T* Ain = buf ? jb.A[1] : jb.A[0];
+  T* const Aout = buf ? jb.A[0] : jb.A[1];
+  double* const nsout = buf ? jb.ns[0] : jb.ns[1];
   // this wavefront's column, rows j + lane + 64 t, loaded into registers now so that the
   // loads overlap the pivot search (positions j and p are reloaded once p is known)
   const int c = j + wgl * 4 + wave;
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
       wi += __shfl_xor(wi, off);
     }
     const T w((R)wr, (R)wi);
-    T* ao = jb.A[buf ^ 1] + (size_t)c * m;
+    T* ao = Aout + (size_t)c * m;
 #pragma unroll
     for (int t = 0; t < RQ_RC; ++t) {
       const int i = j + lane + 64 * t;
@@ -315,12 +319,12 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
       const T d = ar[0] - vr[0] * w;
       const double dd = sqrt((double)d.re * d.re + (double)d.im * d.im);
       const double nsrc = (src >= mlo && src < mhi) ? 0.0 : nsi[src];
-      jb.ns[buf ^ 1][c] = nsrc - dd * dd;
+      nsout[c] = nsrc - dd * dd;
     }
   } else if (c < n) {
     const int src = (c == j) ? p : (c == p ? j : c);
     const T* a = Ain + (size_t)src * m;
-    T* ao = jb.A[buf ^ 1] + (size_t)c * m;
+    T* ao = Aout + (size_t)c * m;
     double ar = 0, ai = 0;
 #pragma unroll 4
     for (int i = j + lane; i < m; i += 64) {
@@ -342,7 +346,7 @@ __global__ void __launch_bounds__(256) rrqr_step_kernel(RRQRJobs<T> jobs, int it
         const T d = a[j] - vel(j) * w;
         const double dd = sqrt((double)d.re * d.re + (double)d.im * d.im);
         const double nsrc = (src >= mlo && src < mhi) ? 0.0 : nsi[src];
-        jb.ns[buf ^ 1][c] = nsrc - dd * dd;
+        nsout[c] = nsrc - dd * dd;
       }
     }
   }
