@@ -413,7 +413,8 @@ __global__ void __launch_bounds__(1024) lu_panel_kernel(T* A, int lda, int n, in
       T* cq = A + (size_t)(j + tid) * lda;
       T a = cq[c], b = cq[p];
       if (p != c) { cq[c] = b; cq[p] = a; }
-      urow[tid] = (p != c) ? b : a;
+      urow[tid] = b;  // b is row p's entry, which is row c's when p == c (no select: a select
+                      // of the two complex values was built in scratch memory)
     }
     __syncthreads();
     const T piv = urow[jj];
