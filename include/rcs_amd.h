@@ -49,7 +49,7 @@ int rcs_mesh_locations(rcs_mesh_t m, double* xyz);
  * E:129-209 and getIndexOfThirdNode E:353-373), plus the edge lengths. */
 int rcs_mesh_rwg(rcs_mesh_t m, int* out6, double* length);
 /* Patch count and patchList (include/Patch.hpp; centroid, area and largestEdge from
- * E:100-124, the reference's largestEdge looking at two of the three edges): out5 holds
+ * E:72-95, the reference's largestEdge E:247-254 looking at two of the three edges): out5 holds
  * centroid x y z, area, largestEdge per patch; conductor (may be NULL) the conductor index. */
 int rcs_mesh_npatch(rcs_mesh_t m);
 int rcs_mesh_patches(rcs_mesh_t m, double* out5, int* conductor);
