@@ -181,12 +181,15 @@ class kernel {
 
  protected:
   kernel() = default;
+  // hostOnly: host tables only (rcs_mesh_create_host), for post-processing without a GPU;
+  // entries, ACA and AIFMM then report an error.
   void create(const std::vector<double>& xyz, const std::vector<int>& tris, int kind, double freq,
-              const std::vector<int>* conductor = nullptr) {
+              const std::vector<int>* conductor = nullptr, bool hostOnly = false) {
     freq_ = freq;
     const int nt = (int)tris.size() / 3;
     const int* cond = (conductor && (int)conductor->size() == nt) ? conductor->data() : nullptr;
-    check(rcs_mesh_create(xyz.data(), (int)xyz.size() / 3, tris.data(), nt, cond, kind, freq, &mesh_));
+    check((hostOnly ? rcs_mesh_create_host : rcs_mesh_create)(xyz.data(), (int)xyz.size() / 3, tris.data(), nt, cond,
+                                                               kind, freq, &mesh_));
     N = rcs_mesh_size(mesh_);
     nodeList.assign(1, Node());
     for (size_t i = 0; i + 2 < xyz.size(); i += 3) nodeList.push_back(Node(xyz[i], xyz[i + 1], xyz[i + 2]));
@@ -226,11 +229,11 @@ class kernel {
 // EFIEMatrix (include/EFIEMatrix.hpp:29-118): RWG EFIE operator of a closed PEC mesh.
 class EFIEMatrix : public kernel {
  public:
-  EFIEMatrix(inputsToKernelClass in, double freq) {
+  EFIEMatrix(inputsToKernelClass in, double freq, bool hostOnly = false) {
     std::vector<double> xyz;
     std::vector<int> tris, cond;
     readMeshFiles(in.nodeListFilename, in.patchListFilename, xyz, tris, &cond);
-    create(xyz, tris, RCS_KERNEL_EFIE, freq, &cond);
+    create(xyz, tris, RCS_KERNEL_EFIE, freq, &cond, hostOnly);
     buildEdgeMap();
   }
   EFIEMatrix(const std::vector<double>& xyz, const std::vector<int>& tris, double freq) {
@@ -293,11 +296,11 @@ class EFIEMatrix : public kernel {
 // single layer C0 * int 1/R at patch centroids.
 class ElectrostaticsMatrix : public kernel {
  public:
-  explicit ElectrostaticsMatrix(inputsToKernelClass in) {
+  explicit ElectrostaticsMatrix(inputsToKernelClass in, bool hostOnly = false) {
     std::vector<double> xyz;
     std::vector<int> tris, cond;
     readMeshFiles(in.nodeListFilename, in.patchListFilename, xyz, tris, &cond);
-    create(xyz, tris, RCS_KERNEL_LAPLACE, 1.0e9, &cond);
+    create(xyz, tris, RCS_KERNEL_LAPLACE, 1.0e9, &cond, hostOnly);
   }
   // getRhs(conductorIndex, rhs) (src/ElectrostaticsMatrix.cpp:374-381): 1 on the patches of
   // that conductor, 0 elsewhere.

@@ -175,3 +175,47 @@ def test_cpp_header_offers_the_reference_surface():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                         src], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def test_cpp_header_postprocessing_on_host_only_meshes(fixture78, tmp_path):
+    """tests/cpp/host_postprocess.cpp runs the reference driver's mesh and post-processing
+    flow through the C++ drop-in classes (host-only meshes, linked against librcs_amd.so, no
+    GPU): the RWG midpoints it builds from edge2PatchMap and nodeList, the right-hand side,
+    getJAtCentroids and getRCS_OnePointQuadrature, and the ElectrostaticsMatrix's getRhs /
+    getCapacitance / getCentroids, each against the oracle."""
+    import subprocess
+    from oracle import pyoracle as O
+    from rcs_amd import mesh as M
+    nodes, tris = fixture78
+    n, p = tmp_path / "nodeList.txt", tmp_path / "patchList.txt"
+    M.write_reference_files(nodes, tris, n, p)
+    exe = tmp_path / "host_postprocess"
+    libdir = os.path.join(ROOT, "rcs_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "host_postprocess.cpp"), "-L", libdir, "-lrcs_amd",
+                        f"-Wl,-rpath,{libdir}", "-o", str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = str(tmp_path / "pp")
+    r = subprocess.run([str(exe), str(n), str(p), out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o = O.Mesh(nodes, tris)
+    N, ncond, npatch = map(int, open(out + "_info.txt").read().split())
+    assert (N, ncond, npatch) == (o.n_rwg, 1, o.n_patch)
+    assert np.allclose(np.loadtxt(out + "_mid.txt"), o.rwg_midpoints(), rtol=0, atol=1e-15)
+    cplx = lambda f: (lambda a: a[:, 0] + 1j * a[:, 1])(np.loadtxt(f, ndmin=2))
+    rhs = cplx(out + "_rhs.txt")
+    ro = o.rhs(1e9)
+    assert np.linalg.norm(rhs - ro) < 1e-13 * np.linalg.norm(ro)
+    coef = 0.001 * (np.arange(N) + 1) + 1j * (0.5 - 0.0007 * np.arange(N))
+    J = cplx(out + "_J.txt").reshape((npatch, 3), order="F")
+    Jo = o.j_at_centroids(1e9, coef)
+    assert np.linalg.norm(J - Jo) < 1e-13 * np.linalg.norm(Jo)
+    th_s = np.loadtxt(out + "_rcs.txt")
+    so = o.rcs(1e9, coef, th_s[:, 0], np.zeros(len(th_s)))
+    assert np.linalg.norm(th_s[:, 1] - so) < 1e-12 * np.linalg.norm(so)
+    lap = open(out + "_lap.txt").read().split("\n")
+    nl, ones, cap = lap[0].split()
+    assert int(nl) == npatch and float(ones) == npatch  # conductor 0 everywhere: rhs all ones
+    pa = o.patches()
+    assert abs(float(cap) - pa[:, 3].sum()) < 1e-14 * pa[:, 3].sum()
+    assert np.allclose(np.loadtxt(lap[1:]), pa[:, :3], rtol=0, atol=1e-15)
