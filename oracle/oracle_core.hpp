@@ -4,6 +4,17 @@
 // reference answer for the HIP implementation in rcs_amd/: only tests/, __graft_entry__.smoke()
 // and bench.py's cpu_baseline leg may load it, never the product path.
 //
+// How it is pinned (tests/test_oracle.py; the reference cannot be built here and ships no
+// numbers): RWG numbering on the reference's 78-patch fixture; panel integrals, the EFIE
+// integration primitives and the Laplace entries against brute-force quadrature in every
+// distance branch; the EFIE operator against the Mie series; the Laplace operator against the
+// sphere's capacitance; the ACA stopping rule by hand-worked known answers and exact-rank
+// recovery; the partitioned RRQR against LAPACK's pivoted QR and hand-worked thresholds; the
+// elimination and solve against dense LU at tight tolerance; the tree statistics against
+// other views of the same tree. Parity unpinned: bit-level rounding and pivot choices at
+// exact ties of the reference's Eigen calls (PartialPivLU, ColPivHouseholderQR's maxPivot,
+// std::sort's tie order in the RRQR); here the first maximal element is taken.
+//
 // Dense types and the Eigen operations the reference relies on (column-major complex
 // matrices, PartialPivLU with first-max partial pivoting, products, adjoints, norms).
 #pragma once
