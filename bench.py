@@ -174,6 +174,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank training and ablation")
     ap.add_argument("--mixed-tol", type=int, default=4, help="TOL_POW of the loose factorizations refined by GMRES")
     ap.add_argument("--plain-tol-accurate", type=int, default=9, help="TOL_POW of the accurate plain FP64 run")
+    ap.add_argument("--extra-fp64-tols", default="", metavar="T,T",
+                    help="more FP64 refined forms at these TOL_POW values (e.g. 3,2), after the standard ones")
     ap.add_argument("--headline", action="store_true",
                     help="only the refined-forms comparison on the 122,880-unknown sphere at 3.82 GHz")
     ap.add_argument("--scatterers", type=int, default=0, metavar="S",
@@ -364,6 +366,12 @@ def learned_rank(R, M, own_log, log):
     return net, res
 
 
+def extra_fp64_forms(R, args):
+    """FP64 refined forms at the --extra-fp64-tols tolerances."""
+    tols = [int(t) for t in args.extra_fp64_tols.split(",") if t.strip()]
+    return [(f"fp64_tol{t}_refined", R.C128, t, False) for t in tols if t != args.mixed_tol]
+
+
 def refined_forms(R, g, bb, b, forms, op_ref, op_check, log, net, sols=None):
     """Each form: factorization (precision, TOL_POW, with or without the rank predictor)
     refined by flexible GMRES against the FP64 operator op_ref to a relative residual of
@@ -441,6 +449,7 @@ def accuracy_and_refined(R, g, bb, b, x_plain, args, log, net):
     res["refinement_operator"] = {"tol_pow": 8, "assembly_s": time.perf_counter() - t0}
     forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
              (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, net is not None)]
+    forms += extra_fp64_forms(R, args)
     res["refined"] = refined_forms(R, g, bb, b, forms, opm, op, log, net, {})
     del opm, op
     return res
@@ -470,6 +479,7 @@ def headline(args):
     forms = [("plain_fp64_refined", R.C128, 6, False), (f"fp64_tol{args.mixed_tol}_refined", R.C128, args.mixed_tol, False),
              (f"c64_tol{args.mixed_tol}_learned_refined", R.C64, args.mixed_tol, True),
              (f"c64_tol{args.mixed_tol}_refined", R.C64, args.mixed_tol, False)]
+    forms += extra_fp64_forms(R, args)
     sols = {}
     ref = refined_forms(R, g, bb, b, forms, opm, None, log, net, sols)
     del opm

@@ -32,3 +32,13 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "exactly one GPU" in r.stderr
+
+
+def test_extra_fp64_forms():
+    import argparse
+
+    class FakeR:
+        C128 = 0
+    a = argparse.Namespace(extra_fp64_tols="3, 2,4", mixed_tol=4)
+    assert bench.extra_fp64_forms(FakeR, a) == [("fp64_tol3_refined", 0, 3, False), ("fp64_tol2_refined", 0, 2, False)]
+    assert bench.extra_fp64_forms(FakeR, argparse.Namespace(extra_fp64_tols="", mixed_tol=4)) == []
