@@ -42,3 +42,10 @@ def test_extra_fp64_forms():
     a = argparse.Namespace(extra_fp64_tols="3, 2,4", mixed_tol=4)
     assert bench.extra_fp64_forms(FakeR, a) == [("fp64_tol3_refined", 0, 3, False), ("fp64_tol2_refined", 0, 2, False)]
     assert bench.extra_fp64_forms(FakeR, argparse.Namespace(extra_fp64_tols="", mixed_tol=4)) == []
+
+
+def test_reference_map_walk_reports_both_costs():
+    r = bench.reference_map_walk(2, 0.15, 1e9, samples=200)
+    assert r["N"] == 3 * 20 * 4 ** 2 // 2 and r["samples"] == 200
+    assert r["lookup_us_per_entry"] > 0 and r["port_entry_us"] > 0
+    assert abs(r["lookup_over_entry"] - r["lookup_us_per_entry"] / r["port_entry_us"]) < 1e-9
