@@ -339,7 +339,8 @@ def learned_rank(R, M, own_log, log):
     nodes, tris = M.perturbed_ellipsoid(6, (0.15, 0.12, 0.10), 0.02, seed=20261203)
     g = R.Mesh(nodes, tris, R.EFIE, freq)
     bb = M.bounding_box(g.locations())
-    res["workload"] = f"perturbed ellipsoid (0.15, 0.12, 0.10) m, 2% radial noise, N={g.N}, f={freq:g} Hz, TOL 6"
+    res["workload"] = (f"perturbed ellipsoid (0.15, 0.12, 0.10) m, 2% radial noise (seed 20261203), N={g.N}, "
+                       f"f={freq:g} Hz, TOL 6; predictor trained with seed 1")
     runs = {}
     for name, model in (("adaptive", None), ("learned", net)):
         log(f"learned rank: assembling the perturbed ellipsoid ({name})")
@@ -536,7 +537,7 @@ def scatterers(args):
     wall = time.perf_counter() - t0
     ns = [r[0] for r in res]
     print(json.dumps({
-        "workload": f"{len(res)} seeded ellipsoids (semi-axes in [0.05, 0.3] m, subdivision 3-5, "
+        "workload": f"{len(res)} seeded ellipsoids (seed 2026; semi-axes in [0.05, 0.3] m, subdivision 3-5, "
                     f"N = {min(ns)}..{max(ns)}, total {sum(ns)}), EFIE 1 GHz, complex64 TOL {args.mixed_tol} + "
                     "FP64 TOL 8 operator + flexible GMRES to 1e-6, 8 plane waves each, one after another",
         "wall_s": wall, "scatterers_per_s": len(res) / wall, "rhs_per_s": 8 * len(res) / wall,
