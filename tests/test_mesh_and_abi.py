@@ -219,3 +219,15 @@ def test_cpp_header_postprocessing_on_host_only_meshes(fixture78, tmp_path):
     pa = o.patches()
     assert abs(float(cap) - pa[:, 3].sum()) < 1e-14 * pa[:, 3].sum()
     assert np.allclose(np.loadtxt(lap[1:]), pa[:, :3], rtol=0, atol=1e-15)
+
+
+def test_c_header_is_plain_c(tmp_path):
+    """include/rcs_amd.h is usable from C (C99, pedantic): the boundary takes plain pointers and
+    sizes, so any FFI that reads C declarations can bind it."""
+    import subprocess
+    src = tmp_path / "c_check.c"
+    src.write_text('#include "rcs_amd.h"\nint main(void) { rcs_mesh_t m = 0; rcs_solver_t s = 0; (void)m; (void)s;'
+                   ' return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
