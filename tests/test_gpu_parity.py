@@ -506,3 +506,30 @@ def test_full_size_sphere_properties():
     xr, it, rr = s.refine(op, b, tol=1e-6, maxit=30, restart=30)
     print(f"direct relres {rr0:.2e}; refined {rr:.1e} after {it} iterations")
     assert rr0 < 1e-2 and rr <= 1e-6 and it <= 10
+
+
+def test_aifmm_variant_t3_matches_oracle():
+    """Variant 1 (T3 parameters: eta = sqrt(3), three levels, no fill-in rank cap) on the
+    elongated ellipsoid, Laplace at TOL 10, against the oracle: the same tree, and either the
+    same ranks and the same solution to 1e-8, or (a cut moved by rounding at a near-tie) a
+    solution as close to the dense LU solution as the oracle's."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    g = R.Mesh(nodes, tris, R.LAPLACE, 1e9)
+    o = O.Mesh(nodes, tris)
+    loc = g.locations()
+    bb = M.bounding_box(loc)
+    b = np.ones(g.N, dtype=complex)
+    sg = R.AIFMM(g, bb, 64, 10, 10, 0, R.C128, 1)
+    sg.factorize()
+    xg = sg.solve(b)
+    so = O.AIFMM(o, O.LAPLACE, 1e9, loc, bb, 64, 10, 10, 0, 1)
+    so.factorize()
+    xo = so.solve(b)
+    st, oi = sg.stats(), so.info()
+    assert (st["nLevels"], st["startLevel"]) == (oi["nLevels"], oi["startLevel"]) == (3, 2)
+    dr = int(np.abs(sg.ranks(3).astype(int) - so.ranks(3).astype(int)).max())
+    if dr == 0:
+        assert _rel(xg, xo) < 1e-8
+    else:
+        xd = o.dense_solve(O.LAPLACE, 1e9, b)
+        assert _rel(xg, xd) < 2 * _rel(xo, xd) + 1e-12

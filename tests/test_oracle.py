@@ -496,3 +496,22 @@ def test_dense_threads_do_not_change_results(rng):
         O.set_dense_threads(old)
     assert np.array_equal(X1, X8)
     assert np.linalg.norm(A @ X1 - B) < 1e-10 * np.linalg.norm(B)
+
+
+def test_aifmm_variant_t3_converges_to_dense():
+    """Variant 1 (the T3 parameters: eta = sqrt(3), at least three levels, no fill-in rank
+    cap) on the elongated ellipsoid: three levels instead of two, RRQR recompressions run, and
+    the solution approaches the dense LU solution as the tolerance tightens."""
+    nodes, tris = M.ellipsoid(3, (0.3, 0.06, 0.06))
+    m = O.Mesh(nodes, tris)
+    loc = m.centroids()
+    b = np.ones(len(loc), dtype=complex)
+    xd = m.dense_solve(O.LAPLACE, 1e9, b)
+    errs = []
+    for tp in (5, 10):
+        s = O.AIFMM(m, O.LAPLACE, 1e9, loc, M.bounding_box(loc), 64, tp, tp, 0, 1)
+        s.factorize()
+        inf = s.info()
+        assert inf["nLevels"] == 3 and inf["n_rrqr"] > 0
+        errs.append(np.linalg.norm(s.solve(b) - xd) / np.linalg.norm(xd))
+    assert errs[0] < 5e-3 and errs[1] < 3e-5 and errs[1] < errs[0] / 10
